@@ -1,0 +1,117 @@
+/*
+ * C ABI of the MI355X (gfx950) spiking step. Plain pointers and sizes; no C++ or torch types.
+ *
+ * Status codes: 0 ok, negative on error (see SNN_E*); snn_last_error() returns a thread-local
+ * message for the last failing call on this thread. No C++ exception crosses this boundary.
+ *
+ * Threading: one host thread per handle at a time; distinct handles are independent. Each handle
+ * owns a HIP stream on its device (default priority).
+ */
+#ifndef SNNCUDA_HIP_H
+#define SNNCUDA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SNN_OK 0
+#define SNN_EINVAL (-1)     /* invalid argument or unsupported network shape */
+#define SNN_EDEVICE (-2)    /* HIP runtime error */
+#define SNN_ECAPACITY (-3)  /* a device capacity was exceeded during a run; results are invalid */
+#define SNN_ENODEVICE (-5)  /* no HIP device */
+
+/* Unchanged reference symbols (reference include/snncuda/c/SNNCudaC.h:7-8, src/c/SNNCudaC.cpp:5-11):
+ * the library version string (static storage) and the compile-time accelerator flag (1: this
+ * library is built with its device backend). */
+const char* snncuda_version(void);
+int snncuda_cuda_available(void);
+
+/* Number of visible HIP devices (0 when none); a device probe, unlike snncuda_cuda_available. */
+int snn_device_count(void);
+const char* snn_last_error(void);
+
+/* A flattened Connectome in connectome order (reference include/snncuda/declarative/Connectome.h:13-37
+ * and NetworkIR.h:14-19). Neuron i is NeuronId i+1, synapse j is SynapseId j+1; source/target are
+ * neuron indices. code_ptr (n_synapses+1 entries, may be NULL) indexes codes; an empty list means
+ * {0}, as the reference's initialize_synapse does. Compartment 0..3 = Soma, Basal, Apical,
+ * Inhibitory; receptor 0..3 = Ampa, Nmda, GabaA, GabaB. All arrays are copied; the caller keeps
+ * ownership. */
+typedef struct snn_net_desc {
+    uint64_t n_neurons;
+    const float* threshold;
+    const uint64_t* window;
+    const float* sim_thr;
+    const uint64_t* max_refs;
+    uint64_t n_synapses;
+    const uint32_t* source;
+    const uint32_t* target;
+    const float* weight;
+    const float* max_weight;
+    const uint32_t* delay;
+    const uint8_t* compartment;
+    const uint8_t* receptor;
+    const uint8_t* plastic;
+    const uint64_t* code_ptr;
+    const uint32_t* codes;
+} snn_net_desc;
+
+/* Capacities (0 = default). Exceeding one during a run returns SNN_ECAPACITY: events are never
+ * dropped silently. */
+typedef struct snn_options {
+    uint64_t max_events_per_tick;   /* delivered events in one tick (<= 2^28) */
+    uint64_t fire_log_capacity;     /* fires held for the delay window */
+    uint64_t raster_capacity;       /* fires recorded for snn_net_read_raster over the handle's life */
+    int32_t track_pre_count;        /* keep per-synapse delivered counts (pre_spike_count) */
+    int32_t device;                 /* HIP device ordinal */
+} snn_options;
+
+typedef struct snn_net snn_net;
+
+/* Replaces CudaBackend::run_resident_propagation's host pack + upload
+ * (reference src/backends/cuda/CudaBackend.cu:3382-3512) and NetworkPropagator's constructor
+ * (src/runtime/NetworkPropagator.cpp:5-29): re-lays the connectome out on the device. */
+int snn_net_create(const snn_net_desc* desc, const snn_options* opts, snn_net** out);
+int snn_net_destroy(snn_net* net);
+
+/* NetworkPropagator::inject (src/runtime/NetworkPropagator.cpp:31-37), n events in call order. */
+int snn_net_inject(snn_net* net, const uint32_t* neuron, const uint64_t* tick, const float* weight, uint64_t n);
+
+/* NetworkPropagator::run_until (src/runtime/NetworkPropagator.cpp:39-43): process every tick up to
+ * and including inclusive_tick. Synchronous; the per-tick loop itself never waits on the host. */
+int snn_net_run_until(snn_net* net, uint64_t inclusive_tick);
+
+/* delivered_spike_count / fired_spike_count (NetworkPropagator.h:27-28) and the next tick. */
+int snn_net_counts(snn_net* net, uint64_t* delivered, uint64_t* fired, uint64_t* next_tick);
+
+/* Per-tick delivered / fired counts for ticks [first_tick, first_tick + n). */
+int snn_net_read_tick_counts(snn_net* net, uint64_t first_tick, uint64_t n, uint64_t* delivered, uint64_t* fired);
+
+/* Fires in processing order (tick-major; within a tick in the reference's order). *n_out receives
+ * the number recorded; neurons (may be NULL) receives min(n_out, cap) neuron indices. */
+int snn_net_read_raster(snn_net* net, uint64_t* n_out, uint32_t* neurons, uint64_t cap);
+
+/* Final neuron state (CudaPropagationResult fields, reference include/snncuda/backends/CudaBackend.h:40-53,
+ * plus the compartment currents/taus of NeuronState). Any pointer may be NULL. currents and taus are
+ * [4][n] (soma, basal, apical, inhibitory). last_fired is -1 when the neuron never fired. */
+int snn_net_read_neurons(snn_net* net, float* membrane, float* currents, float* taus, uint64_t* spike_count,
+                         int64_t* last_fired, uint64_t* temporal_match, uint64_t* temporal_learned);
+
+/* Final synapse state in connectome order (SynapseRuntimeState fields). Any pointer may be NULL.
+ * last_pre / last_post are -1 when unset. pre_count needs track_pre_count. */
+int snn_net_read_synapses(snn_net* net, float* weight, uint64_t* pre_count, uint64_t* update_count,
+                          int64_t* last_pre, int64_t* last_post, uint64_t* code_match, uint64_t* code_learned);
+
+/* Mean duration (ms) of each step kernel over the last run, measured with HIP events on the
+ * handle's stream when profiling is enabled (snn_net_set_profiling). names receives a
+ * comma-separated list. Returns the number of kernels written. */
+int snn_net_set_profiling(snn_net* net, int enabled);
+int snn_net_kernel_times(snn_net* net, double* ms_total, uint64_t* launches, int cap, char* names, uint64_t names_cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

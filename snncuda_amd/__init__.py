@@ -1,0 +1,261 @@
+"""snncuda_amd — MI355X (gfx950) backend for SNNCuda's per-tick spiking step.
+
+Python access to the C ABI in ``include/snncuda_hip.h`` (``libsnncuda_hip.so`` in this package
+directory, built by ``__graft_entry__.build()``). Mirrors the reference's CPU operator interface
+(``NetworkPropagator``: inject / run_until / counters / state readers,
+reference ``include/snncuda/runtime/NetworkPropagator.h:14-47``) and its ctypes module
+(``python/snncuda/__init__.py``: ``version()``, ``cuda_available()``).
+
+There is no CPU fallback: without the built library every entry point raises
+``SNNCudaLibraryError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+PKG = pathlib.Path(__file__).resolve().parent
+LIB_NAME = "libsnncuda_hip.so"
+
+SNN_OK, SNN_EINVAL, SNN_EDEVICE, SNN_ECAPACITY, SNN_ENODEVICE = 0, -1, -2, -3, -5
+
+
+class SNNCudaLibraryError(RuntimeError):
+    """The native library could not be loaded."""
+
+
+class SNNError(RuntimeError):
+    """A C-ABI call failed; ``code`` is the negative status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class CapacityError(SNNError):
+    """A device capacity was exceeded during a run (results of that run are invalid)."""
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("n_neurons", ctypes.c_uint64), ("threshold", ctypes.c_void_p), ("window", ctypes.c_void_p),
+                ("sim_thr", ctypes.c_void_p), ("max_refs", ctypes.c_void_p), ("n_synapses", ctypes.c_uint64),
+                ("source", ctypes.c_void_p), ("target", ctypes.c_void_p), ("weight", ctypes.c_void_p),
+                ("max_weight", ctypes.c_void_p), ("delay", ctypes.c_void_p), ("compartment", ctypes.c_void_p),
+                ("receptor", ctypes.c_void_p), ("plastic", ctypes.c_void_p), ("code_ptr", ctypes.c_void_p),
+                ("codes", ctypes.c_void_p)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("max_events_per_tick", ctypes.c_uint64), ("fire_log_capacity", ctypes.c_uint64),
+                ("raster_capacity", ctypes.c_uint64), ("track_pre_count", ctypes.c_int32),
+                ("device", ctypes.c_int32)]
+
+
+def library_path() -> pathlib.Path:
+    return pathlib.Path(os.environ.get("SNNCUDA_HIP_LIBRARY", PKG / LIB_NAME))
+
+
+def _load() -> ctypes.CDLL:
+    path = library_path()
+    try:
+        L = ctypes.CDLL(str(path))
+    except OSError as err:
+        raise SNNCudaLibraryError(
+            f"cannot load {path}: {err}. Build it with `python -c 'import __graft_entry__ as g; g.build()'`."
+        ) from err
+    P, U64, I = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+    L.snncuda_version.restype = ctypes.c_char_p
+    L.snncuda_cuda_available.restype = I
+    L.snn_device_count.restype = I
+    L.snn_last_error.restype = ctypes.c_char_p
+    L.snn_net_create.argtypes = [ctypes.POINTER(_Desc), ctypes.POINTER(Options), ctypes.POINTER(P)]
+    L.snn_net_destroy.argtypes = [P]
+    L.snn_net_inject.argtypes = [P, P, P, P, U64]
+    L.snn_net_run_until.argtypes = [P, U64]
+    L.snn_net_counts.argtypes = [P, P, P, P]
+    L.snn_net_read_tick_counts.argtypes = [P, U64, U64, P, P]
+    L.snn_net_read_raster.argtypes = [P, P, P, U64]
+    L.snn_net_read_neurons.argtypes = [P] * 8
+    L.snn_net_read_synapses.argtypes = [P] * 8
+    L.snn_net_set_profiling.argtypes = [P, I]
+    L.snn_net_kernel_times.argtypes = [P, P, P, I, ctypes.c_char_p, U64]
+    for name in ("snn_net_create", "snn_net_destroy", "snn_net_inject", "snn_net_run_until", "snn_net_counts",
+                 "snn_net_read_tick_counts", "snn_net_read_raster", "snn_net_read_neurons",
+                 "snn_net_read_synapses", "snn_net_set_profiling", "snn_net_kernel_times"):
+        getattr(L, name).restype = I
+    return L
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def version() -> str:
+    """snncuda_version() (reference python/snncuda/__init__.py:58-59)."""
+    return lib().snncuda_version().decode("utf-8")
+
+
+def cuda_available() -> bool:
+    """snncuda_cuda_available(): compile-time accelerator flag, as in the reference."""
+    return bool(lib().snncuda_cuda_available())
+
+
+def device_count() -> int:
+    return int(lib().snn_device_count())
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc: int) -> None:
+    if rc == SNN_OK:
+        return
+    msg = lib().snn_last_error().decode("utf-8", "replace")
+    if rc == SNN_ECAPACITY:
+        raise CapacityError(rc, msg)
+    raise SNNError(rc, msg)
+
+
+class Network:
+    """A connectome resident on one MI355X, driven like ``NetworkPropagator``.
+
+    ``net`` is any object with the connectome-order arrays of ``snncuda_amd.workloads.Net``.
+    """
+
+    def __init__(self, net, *, max_events_per_tick: int = 0, fire_log_capacity: int = 0,
+                 raster_capacity: int = 0, track_pre_count: bool = True, device: int = 0):
+        L = lib()
+        self._arrays = [np.ascontiguousarray(a, dtype=dt) for a, dt in (
+            (net.threshold, np.float32), (net.window, np.uint64), (net.sim_thr, np.float32),
+            (net.max_refs, np.uint64), (net.src, np.uint32), (net.tgt, np.uint32), (net.weight, np.float32),
+            (net.max_weight, np.float32), (net.delay, np.uint32), (net.comp, np.uint8),
+            (net.receptor, np.uint8), (net.plastic, np.uint8), (net.code_ptr, np.uint64), (net.codes, np.uint32))]
+        a = self._arrays
+        self.n, self.e = int(a[0].shape[0]), int(a[4].shape[0])
+        d = _Desc(self.n, _p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), self.e, *[_p(x) for x in a[4:]])
+        o = Options(max_events_per_tick, fire_log_capacity, raster_capacity, int(track_pre_count), device)
+        h = ctypes.c_void_p()
+        _check(L.snn_net_create(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self._arrays = None  # the library copied everything
+        self.track_pre_count = track_pre_count
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().snn_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # NetworkPropagator::inject / run_until
+    def inject(self, neuron, tick, weight) -> None:
+        nb = np.atleast_1d(np.asarray(neuron, dtype=np.uint32))
+        tk = np.atleast_1d(np.asarray(tick, dtype=np.uint64))
+        wt = np.atleast_1d(np.asarray(weight, dtype=np.float32))
+        nb, tk, wt = np.broadcast_arrays(nb, tk, wt)
+        nb, tk, wt = (np.ascontiguousarray(x) for x in (nb, tk, wt))
+        _check(lib().snn_net_inject(self._h, _p(nb), _p(tk), _p(wt), nb.shape[0]))
+
+    def run_until(self, inclusive_tick: int) -> None:
+        _check(lib().snn_net_run_until(self._h, int(inclusive_tick)))
+
+    def run_commands(self, commands: Sequence) -> None:
+        """Replay (kind, neuron, tick, weight) commands: kind 0 inject, 1 run_until. Consecutive
+        injects are batched into one call (same order)."""
+        batch = []
+        for k, n, t, w in commands:
+            if k == 0:
+                batch.append((n, t, w))
+                continue
+            if batch:
+                arr = np.array(batch, dtype=np.float64)
+                self.inject(arr[:, 0].astype(np.uint32), arr[:, 1].astype(np.uint64), arr[:, 2].astype(np.float32))
+                batch = []
+            self.run_until(t)
+        if batch:
+            arr = np.array(batch, dtype=np.float64)
+            self.inject(arr[:, 0].astype(np.uint32), arr[:, 1].astype(np.uint64), arr[:, 2].astype(np.float32))
+
+    def counts(self):
+        d, f, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().snn_net_counts(self._h, ctypes.byref(d), ctypes.byref(f), ctypes.byref(t)))
+        return d.value, f.value, t.value
+
+    @property
+    def delivered(self) -> int:
+        return self.counts()[0]
+
+    @property
+    def fired(self) -> int:
+        return self.counts()[1]
+
+    def tick_counts(self, first: int = 0, n: Optional[int] = None):
+        if n is None:
+            n = self.counts()[2] - first
+        d = np.zeros(n, np.uint64)
+        f = np.zeros(n, np.uint64)
+        _check(lib().snn_net_read_tick_counts(self._h, first, n, _p(d), _p(f)))
+        return d, f
+
+    def raster(self) -> np.ndarray:
+        """Fires in processing order (tick-major, reference order within a tick)."""
+        n = ctypes.c_uint64()
+        _check(lib().snn_net_read_raster(self._h, ctypes.byref(n), None, 0))
+        out = np.zeros(n.value, np.uint32)
+        _check(lib().snn_net_read_raster(self._h, ctypes.byref(n), _p(out), out.shape[0]))
+        return out
+
+    def neurons(self) -> Dict[str, np.ndarray]:
+        n = self.n
+        o = dict(membrane=np.zeros(n, np.float32), current=np.zeros((4, n), np.float32),
+                 tau=np.zeros((4, n), np.float32), spike_count=np.zeros(n, np.uint64),
+                 last_fired=np.zeros(n, np.int64), t_match=np.zeros(n, np.uint64), t_learned=np.zeros(n, np.uint64))
+        _check(lib().snn_net_read_neurons(self._h, *[_p(o[k]) for k in (
+            "membrane", "current", "tau", "spike_count", "last_fired", "t_match", "t_learned")]))
+        return o
+
+    def synapses(self) -> Dict[str, np.ndarray]:
+        e = self.e
+        o = dict(weight=np.zeros(e, np.float32), upd_count=np.zeros(e, np.uint64), last_pre=np.zeros(e, np.int64),
+                 last_post=np.zeros(e, np.int64), c_match=np.zeros(e, np.uint64), c_learned=np.zeros(e, np.uint64))
+        if self.track_pre_count:
+            o["pre_count"] = np.zeros(e, np.uint64)
+        _check(lib().snn_net_read_synapses(self._h, _p(o["weight"]), _p(o.get("pre_count")), _p(o["upd_count"]),
+                                           _p(o["last_pre"]), _p(o["last_post"]), _p(o["c_match"]),
+                                           _p(o["c_learned"])))
+        return o
+
+    def set_profiling(self, on: bool) -> None:
+        _check(lib().snn_net_set_profiling(self._h, int(on)))
+
+    def kernel_times(self) -> Dict[str, tuple]:
+        """{kernel: (total ms, launches)} over runs since set_profiling(True)."""
+        cap = 16
+        ms = np.zeros(cap, np.float64)
+        nl = np.zeros(cap, np.uint64)
+        names = ctypes.create_string_buffer(512)
+        m = lib().snn_net_kernel_times(self._h, _p(ms), _p(nl), cap, names, 512)
+        keys = names.value.decode().split(",")
+        return {keys[i]: (float(ms[i]), int(nl[i])) for i in range(m)}
+
+
+__all__ = ["SNNCudaLibraryError", "SNNError", "CapacityError", "Network", "Options", "version", "cuda_available",
+           "device_count", "lib", "library_path"]

@@ -1,0 +1,263 @@
+// Host-side re-layout of a flattened Connectome for the device step.
+//
+// Input: neurons and synapses in connectome order (the order ConnectomeBuilder emits, reference
+// src/declarative/Connectome.cpp:87-185), as plain arrays.
+//
+// Output layout (all SoA):
+//   * slots: every synapse gets a slot; slots are sorted by (source, lag of its first code,
+//     connectome index). lag = max(1, delay + code offset): a delay+offset of 0 is delivered on the
+//     next tick because SpikeScheduler clamps late events to the next poll
+//     (src/runtime/SpikeScheduler.cpp:15-19).
+//   * virtual edges (VE): one per (synapse, code offset), sorted by (source, lag, connectome index,
+//     code index). Within one (source, lag) group they are therefore in the order the reference
+//     schedules them (src/runtime/NetworkPropagator.cpp:104-130). When every synapse has exactly
+//     one code, VE i is slot i and no map is stored.
+//   * group table: for source s, grp[s*(lmax+1) + L-1] is the first VE of lag L relative to the
+//     source's row start, grp[s*(lmax+1) + lmax] the row length.
+//   * plastic synapses get a target-major position (tpos) for their STDP record, so a neuron's
+//     post-synaptic scan walks one contiguous range.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "snn_state.h"
+
+namespace snn {
+
+struct NetDesc {
+    uint64_t n_neurons = 0;
+    const float* threshold = nullptr;
+    const uint64_t* window = nullptr;
+    const float* sim_thr = nullptr;
+    const uint64_t* max_refs = nullptr;
+    uint64_t n_synapses = 0;
+    const uint32_t* source = nullptr;
+    const uint32_t* target = nullptr;
+    const float* weight = nullptr;
+    const float* max_weight = nullptr;
+    const uint32_t* delay = nullptr;
+    const uint8_t* compartment = nullptr;
+    const uint8_t* receptor = nullptr;
+    const uint8_t* plastic = nullptr;
+    const uint64_t* code_ptr = nullptr;  // n_synapses + 1, may be null (every synapse has {0})
+    const uint32_t* codes = nullptr;
+};
+
+constexpr uint32_t kMaxLag = 65535;
+
+struct HostNet {
+    uint32_t n = 0;
+    uint64_t e = 0;           // synapses (slots)
+    uint64_t nve = 0;         // virtual edges
+    uint32_t lmax = 1;
+    bool ve_identity = true;  // every synapse has one code
+    bool any_plastic = false;
+    uint32_t expected_all = 2;  // code pattern length if uniform, else 0
+    bool code_compact = true;   // expected == 2 everywhere and every target has max_refs == 1
+
+    std::vector<uint64_t> row;      // n+1, VE row starts
+    std::vector<uint32_t> grp;      // n*(lmax+1)
+    std::vector<uint32_t> ve;       // nve (empty when identity)
+    std::vector<uint32_t> tgt;      // per slot
+    std::vector<float> w;           // per slot
+    std::vector<uint8_t> meta;      // per slot: receptor | compartment << 2 | plastic << 4
+    std::vector<uint16_t> delay;    // per slot (only when any_plastic)
+    std::vector<uint8_t> expected;  // per slot code-pattern length (only when not uniform)
+    std::vector<uint64_t> slot_of;  // per connectome index
+    // STDP
+    std::vector<uint32_t> tpos;     // per slot (kNone when not plastic)
+    std::vector<uint64_t> in_ptr;   // n+1 target-major plastic ranges
+    std::vector<StdpRec> stdp;      // per tpos
+    std::vector<uint64_t> tpos_conn;  // per tpos: connectome index
+    std::vector<float> wmax;        // distinct min(2, max_weight) values (<= 256)
+    // neurons
+    std::vector<NeuronParams> params;
+};
+
+inline uint32_t lag_of(uint32_t delay, uint32_t off) {
+    const uint64_t l = (uint64_t)delay + off;
+    return l == 0 ? 1u : (uint32_t)std::min<uint64_t>(l, 0xFFFFFFFFull);
+}
+
+// Throws std::invalid_argument with a message for inputs outside the device step's limits.
+inline HostNet build_host_net(const NetDesc& d) {
+    HostNet h;
+    if (d.n_neurons == 0) throw std::invalid_argument("network has no neurons");
+    if (d.n_neurons >= 0xFFFFFFFFull) throw std::invalid_argument("more than 2^32-2 neurons");
+    if (d.n_synapses >= 0xFFFFFFFFull) throw std::invalid_argument("more than 2^32-2 synapses on one device");
+    h.n = (uint32_t)d.n_neurons;
+    h.e = d.n_synapses;
+    const uint32_t n = h.n;
+    const uint64_t e = h.e;
+
+    h.params.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (d.max_refs[i] == 0) throw std::invalid_argument("max_reference_patterns must be >= 1");
+        if (d.window[i] > 65535) throw std::invalid_argument("pattern_window_ticks above 65535 is not supported");
+        h.params[i].threshold = d.threshold[i];
+        h.params[i].window = (uint32_t)d.window[i];
+        h.params[i].sim_thr = d.sim_thr[i];
+        h.params[i].max_refs = (uint32_t)std::min<uint64_t>(d.max_refs[i], 0x7FFFFFFFull);
+    }
+
+    // code lists: empty -> {0} (src/runtime/SynapseDendriteProcessor.cpp:68-70)
+    static const uint32_t zero = 0;
+    auto codes_of = [&](uint64_t j, const uint32_t*& p) -> uint32_t {
+        if (d.code_ptr && d.code_ptr[j + 1] > d.code_ptr[j]) {
+            p = d.codes + d.code_ptr[j];
+            return (uint32_t)(d.code_ptr[j + 1] - d.code_ptr[j]);
+        }
+        p = &zero;
+        return 1;
+    };
+
+    uint32_t lmax = 1;
+    uint64_t nve = 0;
+    bool identity = true;
+    uint32_t exp_all = 0;
+    bool exp_uniform = true;
+    for (uint64_t j = 0; j < e; ++j) {
+        if (d.source[j] >= n || d.target[j] >= n) throw std::invalid_argument("synapse endpoint out of range");
+        if (d.compartment[j] > 3 || d.receptor[j] > 3) throw std::invalid_argument("bad compartment or receptor");
+        const uint32_t* c;
+        const uint32_t nc = codes_of(j, c);
+        if (nc > kCodeMax) throw std::invalid_argument("more than 8 spike-code offsets on a synapse");
+        if (nc != 1) identity = false;
+        const uint32_t ex = nc > 2 ? nc : 2;
+        if (j == 0) exp_all = ex;
+        else if (ex != exp_all) exp_uniform = false;
+        for (uint32_t k = 0; k < nc; ++k) {
+            const uint32_t L = lag_of(d.delay[j], c[k]);
+            if (L > kMaxLag) throw std::invalid_argument("delay + code offset above 65535 ticks");
+            lmax = std::max(lmax, L);
+        }
+        nve += nc;
+        if (d.plastic[j]) h.any_plastic = true;
+    }
+    h.lmax = lmax;
+    h.nve = nve;
+    h.ve_identity = identity;
+    h.expected_all = exp_uniform ? (e ? exp_all : 2) : 0;
+    if ((uint64_t)n * (lmax + 1) > (1ull << 31))
+        throw std::invalid_argument("neurons x (max lag + 1) exceeds the group-table limit");
+
+    const uint32_t W = lmax + 1;
+    // slots: counting sort by key (source, lag of code 0), stable in connectome order
+    std::vector<uint64_t> cnt((uint64_t)n * W + 1, 0);
+    std::vector<uint32_t> lag0(e);
+    for (uint64_t j = 0; j < e; ++j) {
+        const uint32_t* c;
+        codes_of(j, c);
+        lag0[j] = lag_of(d.delay[j], c[0]);
+        cnt[(uint64_t)d.source[j] * W + lag0[j] + 1]++;
+    }
+    for (uint64_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
+    h.slot_of.resize(e);
+    h.tgt.resize(e);
+    h.w.resize(e);
+    h.meta.resize(e);
+    if (h.any_plastic) h.delay.resize(e);
+    if (!exp_uniform) h.expected.resize(e);
+    {
+        std::vector<uint64_t> fill(cnt.begin(), cnt.end() - 1);
+        for (uint64_t j = 0; j < e; ++j) {
+            const uint64_t s = fill[(uint64_t)d.source[j] * W + lag0[j]]++;
+            h.slot_of[j] = s;
+            h.tgt[s] = d.target[j];
+            h.w[s] = d.weight[j];
+            h.meta[s] = (uint8_t)(d.receptor[j] | (d.compartment[j] << 2) | ((d.plastic[j] ? 1 : 0) << 4));
+            if (h.any_plastic) h.delay[s] = (uint16_t)std::min<uint32_t>(d.delay[j], 65535);
+            if (!exp_uniform) {
+                const uint32_t* c;
+                const uint32_t nc = codes_of(j, c);
+                h.expected[s] = (uint8_t)(nc > 2 ? nc : 2);
+            }
+        }
+    }
+    if (h.any_plastic) {
+        for (uint64_t j = 0; j < e; ++j)
+            if (d.plastic[j] && d.delay[j] > 65535) throw std::invalid_argument("plastic synapse delay above 65535");
+    }
+
+    // virtual edges and group table
+    h.row.assign((uint64_t)n + 1, 0);
+    h.grp.assign((uint64_t)n * W, 0);
+    if (identity) {
+        for (uint32_t s = 0; s < n; ++s) {
+            const uint64_t base = cnt[(uint64_t)s * W];
+            h.row[s] = base;
+            for (uint32_t L = 1; L <= lmax; ++L) h.grp[(uint64_t)s * W + L - 1] = (uint32_t)(cnt[(uint64_t)s * W + L] - base);
+            h.grp[(uint64_t)s * W + lmax] = (uint32_t)(cnt[(uint64_t)(s + 1) * W] - base);
+        }
+        h.row[n] = e;
+    } else {
+        std::vector<uint64_t> vc((uint64_t)n * W + 1, 0);
+        for (uint64_t j = 0; j < e; ++j) {
+            const uint32_t* c;
+            const uint32_t nc = codes_of(j, c);
+            for (uint32_t k = 0; k < nc; ++k) vc[(uint64_t)d.source[j] * W + lag_of(d.delay[j], c[k]) + 1]++;
+        }
+        for (uint64_t k = 1; k < vc.size(); ++k) vc[k] += vc[k - 1];
+        h.ve.resize(nve);
+        std::vector<uint64_t> fill(vc.begin(), vc.end() - 1);
+        for (uint64_t j = 0; j < e; ++j) {
+            const uint32_t* c;
+            const uint32_t nc = codes_of(j, c);
+            for (uint32_t k = 0; k < nc; ++k) h.ve[fill[(uint64_t)d.source[j] * W + lag_of(d.delay[j], c[k])]++] = (uint32_t)h.slot_of[j];
+        }
+        for (uint32_t s = 0; s < n; ++s) {
+            const uint64_t base = vc[(uint64_t)s * W];
+            h.row[s] = base;
+            for (uint32_t L = 1; L <= lmax; ++L) h.grp[(uint64_t)s * W + L - 1] = (uint32_t)(vc[(uint64_t)s * W + L] - base);
+            const uint64_t len = vc[(uint64_t)(s + 1) * W] - base;
+            if (len > 0xFFFFFFFFull) throw std::invalid_argument("a neuron's fan-out x codes exceeds 2^32");
+            h.grp[(uint64_t)s * W + lmax] = (uint32_t)len;
+        }
+        h.row[n] = nve;
+    }
+
+    // compact code record only when every target neuron keeps at most one learned pattern
+    bool all_one = true;
+    for (uint64_t j = 0; j < e && all_one; ++j) all_one = h.params[d.target[j]].max_refs == 1;
+    h.code_compact = exp_uniform && exp_all == 2 && all_one;
+
+    // STDP records, target-major in connectome order
+    h.tpos.assign(e, kNone);
+    if (h.any_plastic) {
+        h.in_ptr.assign((uint64_t)n + 1, 0);
+        for (uint64_t j = 0; j < e; ++j)
+            if (d.plastic[j]) h.in_ptr[d.target[j] + 1]++;
+        for (uint32_t i = 0; i < n; ++i) h.in_ptr[i + 1] += h.in_ptr[i];
+        std::vector<uint64_t> fill(h.in_ptr.begin(), h.in_ptr.end() - 1);
+        h.stdp.resize(h.in_ptr[n]);
+        h.tpos_conn.resize(h.in_ptr[n]);
+        for (uint64_t j = 0; j < e; ++j) {
+            if (!d.plastic[j]) continue;
+            const uint64_t t = fill[d.target[j]]++;
+            h.tpos[h.slot_of[j]] = (uint32_t)t;
+            h.tpos_conn[t] = j;
+            // StdpRule::apply: maximum = std::min(config max 2.0f, synapse max_weight)
+            const float mx = (d.max_weight[j] < 2.0f) ? d.max_weight[j] : 2.0f;
+            uint32_t idx = 0;
+            while (idx < h.wmax.size() && std::memcmp(&h.wmax[idx], &mx, 4) != 0) ++idx;
+            if (idx == h.wmax.size()) {
+                if (idx == 256) throw std::invalid_argument("more than 256 distinct max_weight values on plastic synapses");
+                h.wmax.push_back(mx);
+            }
+            StdpRec r;
+            r.weight = d.weight[j];
+            r.last_pre = kNone;
+            r.last_post = kNone;
+            r.aux = idx;
+            h.stdp[t] = r;
+        }
+    }
+    return h;
+}
+
+}  // namespace snn

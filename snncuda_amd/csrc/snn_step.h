@@ -1,0 +1,406 @@
+// The per-tick step as item functions: each __global__ kernel in snn_step.hip is a grid-stride
+// loop over one of these, and the CPU model of the kernels in tests/ calls them sequentially.
+//
+// Tick T, in the reference's terms (src/runtime/NetworkPropagator.cpp:78-132):
+//   the due events of T are applied in scheduler insertion order. Insertion order is
+//   (stage, kind, position): a fire at tick te schedules its events at stage te+1 (after
+//   pop_due advanced the scheduler), in fire order, each fire's outgoing synapses in connectome
+//   order and each synapse's codes in order; an inject made while the next poll tick is P has
+//   stage P and comes after the events scheduled by the fires of tick P-1
+//   (src/runtime/SpikeScheduler.cpp:15-23). Stage-major, the due list of T is therefore
+//       [injects with P <= T-lmax], then for L = lmax..1: [fires of T-L, lag-L groups],
+//       [injects with P == T-L+1].
+//   Every due event gets its global position g in that list. Each target's events are sorted by g
+//   and applied sequentially (every write while processing an event for target x touches only x and
+//   x's incoming synapses, so targets are independent). Fires are ordered by the g of the event
+//   that triggered them, which is their order in the reference.
+#pragma once
+
+#include <type_traits>
+
+#include "snn_state.h"
+
+#if defined(__HIPCC__)
+#define SNN_ATOMIC_ADD(p, v) atomicAdd((p), (v))
+#define SNN_ATOMIC_OR(p, v) atomicOr((p), (v))
+#else
+template <typename T>
+inline T snn_host_atomic_add(T* p, T v) { T o = *p; *p = o + v; return o; }
+template <typename T>
+inline T snn_host_atomic_or(T* p, T v) { T o = *p; *p = o | v; return o; }
+#define SNN_ATOMIC_ADD(p, v) snn_host_atomic_add((p), (v))
+#define SNN_ATOMIC_OR(p, v) snn_host_atomic_or((p), (v))
+#endif
+
+namespace snn {
+
+constexpr uint64_t kInjFlag = 1ull << 63;
+constexpr uint32_t kGroupInj = 0x80000000u;
+constexpr uint32_t kMaxEventsPerTick = 1u << 28;  // g is packed in 28 bits
+
+// event entries in a target's bin
+struct Entry8 {            // no STDP, code patterns updated edge-parallel
+    uint32_t gk;           // g << 4 | compartment << 2 | receptor
+    float w;
+};
+struct Entry16 {
+    uint32_t gk;
+    float w;               // weight; reused as a scratch word once the event is applied
+    uint32_t tpos;         // STDP record (kNone: not plastic or external)
+    uint32_t slot;         // synapse slot (kNone: external)
+};
+
+struct Group {             // a run of the tick's due list
+    uint32_t first;        // first segment index
+    uint32_t nseg;
+    uint64_t a;            // fire-log index of the first fire, or first inject index
+    uint32_t b;            // lag L, or inject count | kGroupInj
+    uint32_t pad;
+};
+
+// Everything one tick needs. Plain pointers so the same struct is a kernel argument.
+struct Step {
+    // network
+    uint32_t n, lmax;
+    const uint64_t* row;
+    const uint32_t* grp;
+    const uint32_t* ve;          // null when VE == slot
+    const uint32_t* tgt;
+    const float* w;
+    const uint8_t* meta;
+    const uint16_t* delay;
+    const uint8_t* expected;     // null when uniform
+    uint32_t expected_all;
+    CodeRec2* code2;
+    CodeRecN* codeN;
+    uint32_t* pre_count;         // optional
+    const uint32_t* tpos;
+    const uint64_t* in_ptr;
+    StdpRec* stdp;
+    uint32_t* upd_count;
+    const float* wmax;           // wmax table indexed by StdpRec::aux bits 0-7
+    const NeuronParams* np;
+    NeuronDyn* dyn;
+    TemporalHdr* th;
+    Runs* cand;
+    TemporalRefs* refs;
+    uint32_t* last_fired;
+    const float* decay_tab;      // 4 x kDecayTable
+    const float* ltp;            // kStdpTable
+    const float* ltd;
+    // fire ring
+    uint32_t* firelog;
+    uint32_t* firemult;
+    uint64_t fire_cap;
+    uint64_t* tick_fstart;       // ring of lmax+1
+    uint32_t* tick_fcount;
+    // injects of the current run segment
+    const uint32_t* inj_tgt;
+    const float* inj_w;
+    const uint32_t* inj_stage;
+    const uint64_t* inj_ptr;     // per tick of the segment, +1
+    uint32_t inj_t0, inj_nticks;
+    // per-tick work
+    Group* groups;               // 2*lmax+1
+    uint32_t* seg_count;         // [0] segments, [1] events
+    uint64_t seg_cap, ev_cap;
+    uint64_t* seg_v0;
+    uint32_t* seg_n;
+    uint32_t* seg_off;
+    uint32_t* seg_mult;
+    uint32_t* tcnt;
+    uint32_t* toff;
+    uint32_t* rank;
+    void* bins;
+    uint32_t* fire_bits;
+    uint32_t* fire_neuron;
+    uint32_t* fire_mult;
+    uint32_t* fire_word_off;
+    // device state
+    uint32_t* ctl;               // [0] tick, [1] error flags, [2] fires this tick, [3] pad
+    uint64_t* totals;            // [0] delivered, [1] fired, [2] fire-log head, [3] raster fill
+    uint32_t* tick_delivered;    // per tick of the run segment
+    uint32_t* tick_fired;
+    uint32_t* raster;            // optional
+    uint64_t raster_cap;
+};
+
+SNN_HD uint32_t lower_bound_u32(const uint32_t* a, uint64_t lo, uint64_t hi, uint32_t v) {
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return (uint32_t)lo;
+}
+
+// Group gi of tick T (0 <= gi <= 2*lmax): its segment count and payload.
+SNN_HD void group_info(const Step& S, uint32_t T, uint32_t gi, Group& g) {
+    g.nseg = 0;
+    g.a = 0;
+    g.b = 0;
+    uint64_t ia = 0, ib = 0;
+    if (T >= S.inj_t0 && T - S.inj_t0 < S.inj_nticks) {
+        ia = S.inj_ptr[T - S.inj_t0];
+        ib = S.inj_ptr[T - S.inj_t0 + 1];
+    }
+    if (gi == 0) {
+        // injects with stage P <= T - lmax
+        if (ib > ia && T >= S.lmax) {
+            const uint32_t hi = lower_bound_u32(S.inj_stage, ia, ib, T - S.lmax + 1);
+            if (hi > ia) {
+                g.nseg = 1;
+                g.a = ia;
+                g.b = (uint32_t)(hi - ia) | kGroupInj;
+            }
+        }
+        return;
+    }
+    const uint32_t L = S.lmax - (gi - 1) / 2;
+    if (((gi - 1) & 1u) == 0) {
+        if (T >= L) {
+            const uint32_t te = T - L;
+            const uint32_t slot = te % (S.lmax + 1);
+            g.nseg = S.tick_fcount[slot];
+            g.a = S.tick_fstart[slot];
+            g.b = L;
+        }
+    } else if (ib > ia && T + 1 >= L) {
+        const uint32_t P = T - L + 1;
+        const uint32_t lo = lower_bound_u32(S.inj_stage, ia, ib, P);
+        const uint32_t hi = lower_bound_u32(S.inj_stage, lo, ib, P + 1);
+        if (hi > lo) {
+            g.nseg = 1;
+            g.a = lo;
+            g.b = (uint32_t)(hi - lo) | kGroupInj;
+        }
+    }
+}
+
+// Segment q of the tick: VE start (or inject start | kInjFlag), event count, owner multiplicity.
+SNN_HD void segment_info(const Step& S, const Group& g, uint32_t r, uint64_t& v0, uint32_t& cnt,
+                         uint32_t& mult) {
+    if (g.b & kGroupInj) {
+        v0 = g.a | kInjFlag;
+        cnt = g.b & ~kGroupInj;
+        mult = 0;
+        return;
+    }
+    const uint64_t fi = (g.a + r) % S.fire_cap;
+    const uint32_t s = S.firelog[fi];
+    mult = S.firemult[fi];
+    const uint32_t L = g.b;
+    const uint64_t base = (uint64_t)s * (S.lmax + 1);
+    const uint32_t lo = S.grp[base + L - 1];
+    const uint32_t hi = S.grp[base + L];
+    v0 = S.row[s] + lo;
+    cnt = hi - lo;
+}
+
+SNN_HD uint32_t event_target(const Step& S, uint64_t v0, uint32_t i) {
+    if (v0 & kInjFlag) return S.inj_tgt[(v0 & ~kInjFlag) + i];
+    const uint64_t v = v0 + i;
+    const uint32_t slot = S.ve ? S.ve[v] : (uint32_t)v;
+    return S.tgt[slot];
+}
+
+// count pass: rank of the event inside its target's bin (any order; the fold sorts by g)
+SNN_HD void count_event(const Step& S, uint32_t q, uint32_t i) {
+    const uint32_t x = event_target(S, S.seg_v0[q], i);
+    S.rank[S.seg_off[q] + i] = SNN_ATOMIC_ADD(&S.tcnt[x], 1u);
+}
+
+SNN_HD uint32_t code_expected(const Step& S, uint32_t slot) {
+    return S.expected ? S.expected[slot] : S.expected_all;
+}
+
+// scatter pass: write the event into its target's bin; with CODE_HERE the owner segment (first
+// fire of the source this tick) applies the synapse code pattern `mult` times (the code pattern of
+// a synapse depends only on that synapse's own arrival ticks; same-tick arrivals are identical).
+template <bool WIDE, bool CODE_HERE, bool CODE2>
+SNN_HD uint32_t scatter_event(const Step& S, uint32_t q, uint32_t i, uint32_t T) {
+    const uint64_t v0 = S.seg_v0[q];
+    const uint32_t g = S.seg_off[q] + i;
+    uint32_t x, gk, tp = kNone, slot = kNone;
+    float w;
+    uint32_t err = 0;
+    if (v0 & kInjFlag) {
+        const uint64_t k = (v0 & ~kInjFlag) + i;
+        x = S.inj_tgt[k];
+        w = S.inj_w[k];
+        gk = (g << 4) | (kSoma << 2) | kAmpa;  // process_external_input: Soma, Ampa
+    } else {
+        const uint64_t v = v0 + i;
+        slot = S.ve ? S.ve[v] : (uint32_t)v;
+        x = S.tgt[slot];
+        const uint32_t m = S.meta[slot];
+        w = S.w[slot];
+        gk = (g << 4) | (m & 15u);
+        if (WIDE && (m & 16u)) tp = S.tpos[slot];
+        if (CODE_HERE) {
+            const uint32_t mult = S.seg_mult[q];
+            if (mult) {
+                const NeuronParams p = S.np[x];
+                if (CODE2) err |= code_observe2(S.code2[slot], T, mult, p.window, p.sim_thr, p.max_refs);
+                else err |= code_observeN(S.codeN[slot], T, mult, code_expected(S, slot), p.window, p.sim_thr, p.max_refs);
+                if (S.pre_count) S.pre_count[slot] += mult;
+            }
+        }
+    }
+    const uint64_t pos = (uint64_t)S.toff[x] + S.rank[g];
+    if (WIDE) {
+        Entry16 e;
+        e.gk = gk;
+        e.w = w;
+        e.tpos = tp;
+        e.slot = slot;
+        ((Entry16*)S.bins)[pos] = e;
+    } else {
+        Entry8 e;
+        e.gk = gk;
+        e.w = w;
+        ((Entry8*)S.bins)[pos] = e;
+    }
+    return err;
+}
+
+template <typename E>
+SNN_HD void sort_by_g(E* ev, uint32_t k) {
+    // Shell sort (Ciura gaps) in place; k is small for almost every target.
+    const uint32_t gaps[8] = {701, 301, 132, 57, 23, 10, 4, 1};
+    for (uint32_t gi = 0; gi < 8; ++gi) {
+        const uint32_t gap = gaps[gi];
+        if (gap >= k) continue;
+        for (uint32_t i = gap; i < k; ++i) {
+            const E t = ev[i];
+            const uint32_t key = t.gk >> 4;
+            uint32_t j = i;
+            while (j >= gap && (ev[j - gap].gk >> 4) > key) {
+                ev[j] = ev[j - gap];
+                j -= gap;
+            }
+            ev[j] = t;
+        }
+    }
+}
+
+SNN_HD uint32_t f2u(float f) {
+    union { float f; uint32_t u; } c;
+    c.f = f;
+    return c.u;
+}
+SNN_HD float u2f(uint32_t u) {
+    union { float f; uint32_t u; } c;
+    c.u = u;
+    return c.f;
+}
+
+// Bring a record up to `fires` post-synaptic updates of tick T (apply_post_spike_plasticity,
+// SynapseDendriteProcessor.cpp:138-173, once per fire of the target at T).
+SNN_HD uint32_t post_catch_up(const Step& S, StdpRec& r, uint32_t T, uint32_t fires) {
+    uint32_t applied = r.last_post == T ? (r.aux >> 8) : 0u;
+    uint32_t changed = 0;
+    for (; applied < fires; ++applied) {
+        r.last_post = T;
+        if (r.last_pre != kNone && stdp_apply(r, r.last_pre, T, S.ltp, S.ltd, S.wmax)) ++changed;
+    }
+    if (r.last_post == T) r.aux = (r.aux & 0xFFu) | (applied << 8);
+    return changed;
+}
+
+// The sequential fold of target x's events at tick T. Returns error flags.
+template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
+SNN_HD uint32_t fold_neuron(const Step& S, uint32_t x, uint32_t T) {
+    const uint32_t k = S.tcnt[x];
+    if (k == 0) return 0;
+    S.tcnt[x] = 0;
+    typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
+    E* ev = (E*)S.bins + S.toff[x];
+    sort_by_g(ev, k);
+
+    const NeuronParams p = S.np[x];
+    NeuronDyn d = S.dyn[x];
+    TemporalHdr h = S.th[x];
+    uint32_t err = 0;
+    bool cand_dirty = false;
+    decay_to(d, T, S.decay_tab);
+    uint32_t fires = 0, first_g = 0;
+    for (uint32_t i = 0; i < k; ++i) {
+        const E e = ev[i];
+        const uint32_t g = e.gk >> 4;
+        const uint32_t comp = (e.gk >> 2) & 3u;
+        const uint32_t rec = e.gk & 3u;
+        float w = e.w;
+        if constexpr (WIDE) {
+            if (STDP && e.tpos != kNone) {
+                StdpRec r = S.stdp[e.tpos];
+                // first the post-synaptic updates of this tick's earlier fires of x
+                uint32_t changed = post_catch_up(S, r, T, fires);
+                // pre-synaptic side: process_synaptic_input (:95-126); the code offset is ignored
+                const uint32_t dl = S.delay[e.slot];
+                const uint32_t pre = T >= dl ? T - dl : T;
+                if (r.last_post != kNone && stdp_apply(r, pre, r.last_post, S.ltp, S.ltd, S.wmax)) ++changed;
+                r.last_pre = pre;
+                S.stdp[e.tpos] = r;
+                if (changed) S.upd_count[e.tpos] += changed;
+                w = r.weight;
+            }
+            if (!CODE_HERE && e.slot != kNone) {
+                const uint32_t slot = e.slot;
+                if (CODE2) err |= code_observe2(S.code2[slot], T, 1, p.window, p.sim_thr, p.max_refs);
+                else err |= code_observeN(S.codeN[slot], T, 1, code_expected(S, slot), p.window, p.sim_thr, p.max_refs);
+                if (S.pre_count) S.pre_count[slot] += 1;
+            }
+        }
+        // process_input (:175-212)
+        const float applied_current = w * receptor_gain(rec) * receptor_sign(rec);
+        d.cur[comp] += applied_current;
+        d.taus = (d.taus & ~(3u << (2 * comp))) | (rec << (2 * comp));
+        err |= temporal_observe(h, &S.cand[x], &S.refs[x], T, p, cand_dirty);
+        d.membrane = d.cur[0] + d.cur[1] + d.cur[2] + d.cur[3];
+        if (d.membrane >= p.threshold) {
+            d.membrane = 0.0f;
+            d.cur[0] = 0.0f;
+            d.cur[1] = 0.0f;
+            d.cur[2] = 0.0f;
+            d.cur[3] = 0.0f;
+            d.taus = kResetTauCodes;
+            d.spike_count += 1;
+            SNN_ATOMIC_OR(&S.fire_bits[g >> 5], 1u << (g & 31u));
+            S.fire_neuron[g] = x;
+            S.fire_mult[g] = 0;
+            if (fires == 0) first_g = g;
+            ++fires;
+        }
+    }
+    if (fires) {
+        S.fire_mult[first_g] = fires;
+        S.last_fired[x] = T;
+        if constexpr (WIDE && STDP) {
+            // bring the synapses that had events this tick up to all fires; the others are done by
+            // the post-synaptic scan kernel
+            for (uint32_t i = 0; i < k; ++i) {
+                const uint32_t t = ev[i].tpos;
+                if (t == kNone) continue;
+                StdpRec r = S.stdp[t];
+                const uint32_t changed = post_catch_up(S, r, T, fires);
+                S.stdp[t] = r;
+                if (changed) S.upd_count[t] += changed;
+            }
+        }
+    }
+    S.dyn[x] = d;
+    S.th[x] = h;
+    (void)cand_dirty;
+    return err;
+}
+
+// post-synaptic scan of neuron x after m fires at tick T, for one incoming record
+SNN_HD void post_scan_record(const Step& S, uint64_t t, uint32_t T, uint32_t m) {
+    StdpRec r = S.stdp[t];
+    if (r.last_post == T && (r.aux >> 8) == m) return;  // done inside the fold
+    const uint32_t changed = post_catch_up(S, r, T, m);
+    S.stdp[t] = r;
+    if (changed) S.upd_count[t] += changed;
+}
+
+}  // namespace snn

@@ -1,0 +1,74 @@
+"""The HIP step on the MI355X against the reference fixtures and the oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle
+import snncuda_amd as sa
+from snncuda_amd import workloads as W
+from tests import golden_util as G
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _needs_device():
+    if sa.device_count() == 0:
+        pytest.fail("no HIP device visible: the GPU tests must run on an MI355X")
+
+
+def run_device(net, **kw):
+    dev = sa.Network(net, **kw)
+    dev.run_commands(net.commands)
+    return dev
+
+
+@pytest.mark.parametrize("name", sorted(G.CASES))
+def test_device_matches_reference_fixture(name):
+    print(f"case {name}", flush=True)
+    net = G.CASES[name]()
+    g = G.load(name)
+    with run_device(net, max_events_per_tick=1 << 22, fire_log_capacity=1 << 20) as dev:
+        d, f = dev.tick_counts()
+        bad = G.mismatches(g, d, f, dev.raster(), dev.neurons(), dev.synapses())
+        assert dev.delivered == int(g["delivered"]) and dev.fired == int(g["fired"])
+    assert not bad, f"{name}: device differs from the reference in {bad}"
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_device_matches_oracle_other_seeds(seed):
+    print(f"seed {seed}", flush=True)
+    for net in (W.config0(seed, threshold=1.1, ticks=400), W.feature_net(100 + seed),
+                W.config1_reduced(seed, ticks=150)):
+        ref = oracle.Oracle(net)
+        ref.run_commands(net.commands)
+        with run_device(net) as dev:
+            assert np.array_equal(dev.tick_counts()[0], ref.tick_counts()[0]), net.name
+            assert np.array_equal(dev.raster(), ref.fires()[1]), net.name
+            dn, rn = dev.neurons(), ref.neurons()
+            for k in ("membrane", "current", "spike_count", "t_match", "t_learned", "last_fired"):
+                assert np.array_equal(dn[k], rn[k]), (net.name, k)
+            ds, rs = dev.synapses(), ref.synapses()
+            for k in ("weight", "pre_count", "upd_count", "last_pre", "last_post", "c_match", "c_learned"):
+                assert np.array_equal(ds[k], rs[k].astype(ds[k].dtype)), (net.name, k)
+
+
+def test_capacity_overflow_raises():
+    net = G.CASES["config0_burst"]()
+    with pytest.raises(sa.CapacityError):
+        run_device(net, fire_log_capacity=1 << 12)
+    with pytest.raises(sa.CapacityError):
+        run_device(net, max_events_per_tick=1 << 12, fire_log_capacity=1 << 20)
+
+
+def test_split_runs_equal_one_run():
+    net = W.config0(5, threshold=1.1, ticks=300)
+    injects = [c for c in net.commands if c[0] == 0]
+    a = run_device(net)
+    b = sa.Network(net)
+    b.run_commands(injects)
+    for t in (0, 1, 57, 58, 200, 299):
+        b.run_until(t)
+    assert np.array_equal(a.raster(), b.raster())
+    assert np.array_equal(a.synapses()["weight"], b.synapses()["weight"])
+    a.close()
+    b.close()
