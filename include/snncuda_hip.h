@@ -76,6 +76,34 @@ typedef struct snn_net snn_net;
 int snn_net_create(const snn_net_desc* desc, const snn_options* opts, snn_net** out);
 int snn_net_destroy(snn_net* net);
 
+/* No counterpart in the reference: builds a seeded synthetic E/I network directly on the device
+ * (the network the reference would receive as a Connectome; used for benchmarks too large to pack
+ * on the host). Network neurons [0, n) with fan-out fanout[i], then n_sources input neurons with
+ * source_fanout each; the same counter hash as snncuda_amd/workloads.py's ei_network, so the result
+ * equals snn_net_create on that function's arrays. One code offset {0} per synapse. */
+typedef struct snn_ei_desc {
+    uint64_t n;
+    uint64_t n_sources;
+    const uint32_t* fanout;
+    uint32_t source_fanout;
+    uint64_t seed;
+    uint64_t n_exc;
+    float w_exc;
+    float w_inh;
+    uint32_t delay_max;         /* <= 63 */
+    uint32_t n_comps;
+    uint8_t comps[4];
+    int32_t plastic;
+    float threshold;
+    float source_threshold;
+    uint32_t window;
+    float sim_thr;
+    uint32_t max_refs;
+} snn_ei_desc;
+int snn_net_create_ei(const snn_ei_desc* desc, const snn_options* opts, snn_net** out);
+
+
+
 /* NetworkPropagator::inject (src/runtime/NetworkPropagator.cpp:31-37), n events in call order. */
 int snn_net_inject(snn_net* net, const uint32_t* neuron, const uint64_t* tick, const float* weight, uint64_t n);
 

@@ -58,6 +58,15 @@ class Options(ctypes.Structure):
                 ("device", ctypes.c_int32)]
 
 
+class _EiDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("n_sources", ctypes.c_uint64), ("fanout", ctypes.c_void_p),
+                ("source_fanout", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("n_exc", ctypes.c_uint64),
+                ("w_exc", ctypes.c_float), ("w_inh", ctypes.c_float), ("delay_max", ctypes.c_uint32),
+                ("n_comps", ctypes.c_uint32), ("comps", ctypes.c_uint8 * 4), ("plastic", ctypes.c_int32),
+                ("threshold", ctypes.c_float), ("source_threshold", ctypes.c_float), ("window", ctypes.c_uint32),
+                ("sim_thr", ctypes.c_float), ("max_refs", ctypes.c_uint32)]
+
+
 def library_path() -> pathlib.Path:
     return pathlib.Path(os.environ.get("SNNCUDA_HIP_LIBRARY", PKG / LIB_NAME))
 
@@ -76,6 +85,7 @@ def _load() -> ctypes.CDLL:
     L.snn_device_count.restype = I
     L.snn_last_error.restype = ctypes.c_char_p
     L.snn_net_create.argtypes = [ctypes.POINTER(_Desc), ctypes.POINTER(Options), ctypes.POINTER(P)]
+    L.snn_net_create_ei.argtypes = [ctypes.POINTER(_EiDesc), ctypes.POINTER(Options), ctypes.POINTER(P)]
     L.snn_net_destroy.argtypes = [P]
     L.snn_net_inject.argtypes = [P, P, P, P, U64]
     L.snn_net_run_until.argtypes = [P, U64]
@@ -86,7 +96,7 @@ def _load() -> ctypes.CDLL:
     L.snn_net_read_synapses.argtypes = [P] * 8
     L.snn_net_set_profiling.argtypes = [P, I]
     L.snn_net_kernel_times.argtypes = [P, P, P, I, ctypes.c_char_p, U64]
-    for name in ("snn_net_create", "snn_net_destroy", "snn_net_inject", "snn_net_run_until", "snn_net_counts",
+    for name in ("snn_net_create", "snn_net_create_ei", "snn_net_destroy", "snn_net_inject", "snn_net_run_until", "snn_net_counts",
                  "snn_net_read_tick_counts", "snn_net_read_raster", "snn_net_read_neurons",
                  "snn_net_read_synapses", "snn_net_set_profiling", "snn_net_kernel_times"):
         getattr(L, name).restype = I
@@ -133,9 +143,14 @@ class Network:
     ``net`` is any object with the connectome-order arrays of ``snncuda_amd.workloads.Net``.
     """
 
-    def __init__(self, net, *, max_events_per_tick: int = 0, fire_log_capacity: int = 0,
-                 raster_capacity: int = 0, track_pre_count: bool = True, device: int = 0):
+    def __init__(self, net=None, *, max_events_per_tick: int = 0, fire_log_capacity: int = 0,
+                 raster_capacity: int = 0, track_pre_count: bool = True, device: int = 0, spec=None):
         L = lib()
+        o = Options(max_events_per_tick, fire_log_capacity, raster_capacity, int(track_pre_count), device)
+        self.track_pre_count = track_pre_count
+        if spec is not None:
+            self._create_from_spec(spec, o)
+            return
         self._arrays = [np.ascontiguousarray(a, dtype=dt) for a, dt in (
             (net.threshold, np.float32), (net.window, np.uint64), (net.sim_thr, np.float32),
             (net.max_refs, np.uint64), (net.src, np.uint32), (net.tgt, np.uint32), (net.weight, np.float32),
@@ -144,12 +159,23 @@ class Network:
         a = self._arrays
         self.n, self.e = int(a[0].shape[0]), int(a[4].shape[0])
         d = _Desc(self.n, _p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), self.e, *[_p(x) for x in a[4:]])
-        o = Options(max_events_per_tick, fire_log_capacity, raster_capacity, int(track_pre_count), device)
         h = ctypes.c_void_p()
         _check(L.snn_net_create(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
         self._arrays = None  # the library copied everything
-        self.track_pre_count = track_pre_count
+
+    def _create_from_spec(self, spec: dict, o: "Options") -> None:
+        """Generate a workloads.ei_network on the device (snn_net_create_ei)."""
+        fan = np.ascontiguousarray(np.broadcast_to(np.asarray(spec["fanout"], np.uint32), (spec["n"],)))
+        comps = (ctypes.c_uint8 * 4)(*(list(spec["compartments"]) + [0] * 4)[:4])
+        d = _EiDesc(spec["n"], spec["n_sources"], _p(fan), spec["source_fanout"], spec["seed"], spec["n_exc"],
+                    spec["w_exc"], spec["w_inh"], spec["delay_max"], len(spec["compartments"]), comps,
+                    int(spec["plastic"]), spec["threshold"], 1.0, spec["window"], spec["sim_thr"], spec["max_refs"])
+        h = ctypes.c_void_p()
+        _check(lib().snn_net_create_ei(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self.n = int(spec["n"] + spec["n_sources"])
+        self.e = int(fan.astype(np.uint64).sum() + np.uint64(spec["n_sources"]) * np.uint64(spec["source_fanout"]))
 
     def close(self) -> None:
         if getattr(self, "_h", None):

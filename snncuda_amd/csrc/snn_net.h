@@ -66,14 +66,14 @@ struct HostNet {
     std::vector<uint32_t> tgt;      // per slot
     std::vector<float> w;           // per slot
     std::vector<uint8_t> meta;      // per slot: receptor | compartment << 2 | plastic << 4
-    std::vector<uint16_t> delay;    // per slot (only when any_plastic)
+    std::vector<uint16_t> delay;    // per slot
     std::vector<uint8_t> expected;  // per slot code-pattern length (only when not uniform)
-    std::vector<uint64_t> slot_of;  // per connectome index
+    std::vector<uint32_t> slot_of;  // per connectome index
     // STDP
     std::vector<uint32_t> tpos;     // per slot (kNone when not plastic)
     std::vector<uint64_t> in_ptr;   // n+1 target-major plastic ranges
     std::vector<StdpRec> stdp;      // per tpos
-    std::vector<uint64_t> tpos_conn;  // per tpos: connectome index
+    std::vector<uint32_t> tpos_conn;  // per tpos: connectome index
     std::vector<float> wmax;        // distinct min(2, max_weight) values (<= 256)
     // neurons
     std::vector<NeuronParams> params;
@@ -161,17 +161,17 @@ inline HostNet build_host_net(const NetDesc& d) {
     h.tgt.resize(e);
     h.w.resize(e);
     h.meta.resize(e);
-    if (h.any_plastic) h.delay.resize(e);
+    h.delay.resize(e);
     if (!exp_uniform) h.expected.resize(e);
     {
         std::vector<uint64_t> fill(cnt.begin(), cnt.end() - 1);
         for (uint64_t j = 0; j < e; ++j) {
             const uint64_t s = fill[(uint64_t)d.source[j] * W + lag0[j]]++;
-            h.slot_of[j] = s;
+            h.slot_of[j] = (uint32_t)s;
             h.tgt[s] = d.target[j];
             h.w[s] = d.weight[j];
             h.meta[s] = (uint8_t)(d.receptor[j] | (d.compartment[j] << 2) | ((d.plastic[j] ? 1 : 0) << 4));
-            if (h.any_plastic) h.delay[s] = (uint16_t)std::min<uint32_t>(d.delay[j], 65535);
+            h.delay[s] = (uint16_t)std::min<uint32_t>(d.delay[j], 65535);
             if (!exp_uniform) {
                 const uint32_t* c;
                 const uint32_t nc = codes_of(j, c);
@@ -240,7 +240,7 @@ inline HostNet build_host_net(const NetDesc& d) {
             if (!d.plastic[j]) continue;
             const uint64_t t = fill[d.target[j]]++;
             h.tpos[h.slot_of[j]] = (uint32_t)t;
-            h.tpos_conn[t] = j;
+            h.tpos_conn[t] = (uint32_t)j;
             // StdpRule::apply: maximum = std::min(config max 2.0f, synapse max_weight)
             const float mx = (d.max_weight[j] < 2.0f) ? d.max_weight[j] : 2.0f;
             uint32_t idx = 0;

@@ -259,6 +259,18 @@ struct CodeRecN {
     uint16_t ref[kCodeRefs][kCodeMax];
 };
 
+// Tick of the last arrival recorded in a code record, or kNone before the first arrival. Every
+// synaptic event updates its synapse's code pattern, so this is the synapse's last delivery tick.
+SNN_HD uint32_t code_last_arrival2(const CodeRec2& r) {
+    const uint32_t nobs = (r.w1 >> 28) & 3u;
+    if (nobs == 0) return kNone;
+    return r.start + (nobs == 2 ? (r.obs >> 16) : (r.obs & 0xFFFFu));
+}
+SNN_HD uint32_t code_last_arrival(const CodeRecN& r) {
+    const uint32_t nobs = r.flags & 0xFu;
+    return nobs == 0 ? kNone : r.start + r.obs[nobs - 1];
+}
+
 // FP64 cosine of two equal-length u16 sequences (index order)
 SNN_HD float cosine_u16(const uint16_t* l, const uint16_t* r, uint32_t k) {
     double dot = 0.0, ln = 0.0, rn = 0.0;

@@ -108,10 +108,15 @@ struct Step {
     uint32_t* seg_n;
     uint32_t* seg_off;
     uint32_t* seg_mult;
-    uint32_t* tcnt;
-    uint32_t* toff;
-    uint32_t* rank;
-    void* bins;
+    // target partitions: partition p holds neurons [p << pshift, (p+1) << pshift)
+    uint32_t pshift, nparts, nwg;
+    uint32_t* hist;              // [nparts][nwg] per-workgroup event counts, then their bucket bases
+    uint32_t* pgk;               // partition buckets (SoA), event positions [0, D)
+    float* pw;
+    uint16_t* ptl;               // target index within its partition
+    uint32_t* ptpos;             // WIDE only
+    uint32_t* pslot;             // WIDE only
+    void* stage;                 // fold scratch for partitions larger than the LDS stage
     uint32_t* fire_bits;
     uint32_t* fire_neuron;
     uint32_t* fire_mult;
@@ -203,62 +208,48 @@ SNN_HD uint32_t event_target(const Step& S, uint64_t v0, uint32_t i) {
     return S.tgt[slot];
 }
 
-// count pass: rank of the event inside its target's bin (any order; the fold sorts by g)
-SNN_HD void count_event(const Step& S, uint32_t q, uint32_t i) {
-    const uint32_t x = event_target(S, S.seg_v0[q], i);
-    S.rank[S.seg_off[q] + i] = SNN_ATOMIC_ADD(&S.tcnt[x], 1u);
-}
-
 SNN_HD uint32_t code_expected(const Step& S, uint32_t slot) {
     return S.expected ? S.expected[slot] : S.expected_all;
 }
 
-// scatter pass: write the event into its target's bin; with CODE_HERE the owner segment (first
-// fire of the source this tick) applies the synapse code pattern `mult` times (the code pattern of
-// a synapse depends only on that synapse's own arrival ticks; same-tick arrivals are identical).
-template <bool WIDE, bool CODE_HERE, bool CODE2>
-SNN_HD uint32_t scatter_event(const Step& S, uint32_t q, uint32_t i, uint32_t T) {
+// The record of event (q, i): its target x, position key and payload. With CODE_HERE the owner
+// segment (first fire of the source this tick) also applies the synapse code pattern `mult` times:
+// the code pattern of a synapse depends only on that synapse's own arrival ticks, and same-tick
+// arrivals are identical. Returns error flags.
+template <bool WIDE, bool CODE_HERE, bool CODE2, typename E>
+SNN_HD uint32_t make_event(const Step& S, uint32_t q, uint32_t i, uint32_t T, uint32_t& x, E& e) {
     const uint64_t v0 = S.seg_v0[q];
     const uint32_t g = S.seg_off[q] + i;
-    uint32_t x, gk, tp = kNone, slot = kNone;
-    float w;
     uint32_t err = 0;
     if (v0 & kInjFlag) {
         const uint64_t k = (v0 & ~kInjFlag) + i;
         x = S.inj_tgt[k];
-        w = S.inj_w[k];
-        gk = (g << 4) | (kSoma << 2) | kAmpa;  // process_external_input: Soma, Ampa
-    } else {
-        const uint64_t v = v0 + i;
-        slot = S.ve ? S.ve[v] : (uint32_t)v;
-        x = S.tgt[slot];
-        const uint32_t m = S.meta[slot];
-        w = S.w[slot];
-        gk = (g << 4) | (m & 15u);
-        if (WIDE && (m & 16u)) tp = S.tpos[slot];
-        if (CODE_HERE) {
-            const uint32_t mult = S.seg_mult[q];
-            if (mult) {
-                const NeuronParams p = S.np[x];
-                if (CODE2) err |= code_observe2(S.code2[slot], T, mult, p.window, p.sim_thr, p.max_refs);
-                else err |= code_observeN(S.codeN[slot], T, mult, code_expected(S, slot), p.window, p.sim_thr, p.max_refs);
-                if (S.pre_count) S.pre_count[slot] += mult;
-            }
+        e.w = S.inj_w[k];
+        e.gk = (g << 4) | (kSoma << 2) | kAmpa;  // process_external_input: Soma, Ampa
+        if constexpr (WIDE) {
+            e.tpos = kNone;
+            e.slot = kNone;
         }
+        return 0;
     }
-    const uint64_t pos = (uint64_t)S.toff[x] + S.rank[g];
-    if (WIDE) {
-        Entry16 e;
-        e.gk = gk;
-        e.w = w;
-        e.tpos = tp;
+    const uint64_t v = v0 + i;
+    const uint32_t slot = S.ve ? S.ve[v] : (uint32_t)v;
+    x = S.tgt[slot];
+    const uint32_t m = S.meta[slot];
+    e.w = S.w[slot];
+    e.gk = (g << 4) | (m & 15u);
+    if constexpr (WIDE) {
+        e.tpos = (m & 16u) ? S.tpos[slot] : kNone;
         e.slot = slot;
-        ((Entry16*)S.bins)[pos] = e;
-    } else {
-        Entry8 e;
-        e.gk = gk;
-        e.w = w;
-        ((Entry8*)S.bins)[pos] = e;
+    }
+    if (CODE_HERE) {
+        const uint32_t mult = S.seg_mult[q];
+        if (mult) {
+            const NeuronParams p = S.np[x];
+            if (CODE2) err |= code_observe2(S.code2[slot], T, mult, p.window, p.sim_thr, p.max_refs);
+            else err |= code_observeN(S.codeN[slot], T, mult, code_expected(S, slot), p.window, p.sim_thr, p.max_refs);
+            if (S.pre_count) S.pre_count[slot] += mult;
+        }
     }
     return err;
 }
@@ -307,14 +298,11 @@ SNN_HD uint32_t post_catch_up(const Step& S, StdpRec& r, uint32_t T, uint32_t fi
     return changed;
 }
 
-// The sequential fold of target x's events at tick T. Returns error flags.
-template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
-SNN_HD uint32_t fold_neuron(const Step& S, uint32_t x, uint32_t T) {
-    const uint32_t k = S.tcnt[x];
+// The sequential fold of target x's k events at tick T (ev is the target's own array, in any
+// order; it is sorted by g here). Returns error flags.
+template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2, typename E>
+SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32_t T) {
     if (k == 0) return 0;
-    S.tcnt[x] = 0;
-    typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
-    E* ev = (E*)S.bins + S.toff[x];
     sort_by_g(ev, k);
 
     const NeuronParams p = S.np[x];

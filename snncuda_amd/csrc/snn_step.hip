@@ -214,38 +214,109 @@ __global__ void __launch_bounds__(kBlock) k_segments(Step S) {
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_count(Step S) {
+// Per-workgroup histogram of the tick's events over target partitions (LDS atomics only).
+// The segment -> wave assignment is identical in k_pscatter, which replays the same events.
+__global__ void __launch_bounds__(kBlock) k_pcount(Step S) {
+    extern __shared__ uint32_t h[];
+    for (uint32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) h[p] = 0;
+    __syncthreads();
     const uint32_t nseg = S.seg_count[0];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
     for (uint32_t q = wave; q < nseg; q += nwaves) {
         const uint32_t cnt = S.seg_n[q];
-        for (uint32_t i = lane; i < cnt; i += kWave) count_event(S, q, i);
+        const uint64_t v0 = S.seg_v0[q];
+        for (uint32_t i = lane; i < cnt; i += kWave) atomicAdd(&h[event_target(S, v0, i) >> S.pshift], 1u);
     }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) S.hist[(uint64_t)p * S.nwg + blockIdx.x] = h[p];
 }
 
+// Place each event in its partition's bucket at (scanned workgroup base + LDS rank).
 template <bool WIDE, bool CODE_HERE, bool CODE2>
-__global__ void __launch_bounds__(kBlock) k_scatter(Step S) {
+__global__ void __launch_bounds__(kBlock) k_pscatter(Step S) {
+    extern __shared__ uint32_t base[];
+    for (uint32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) base[p] = S.hist[(uint64_t)p * S.nwg + blockIdx.x];
+    __syncthreads();
     const uint32_t T = S.ctl[0];
     const uint32_t nseg = S.seg_count[0];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+    const uint32_t mask = (1u << S.pshift) - 1;
     uint32_t err = 0;
+    typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
     for (uint32_t q = wave; q < nseg; q += nwaves) {
         const uint32_t cnt = S.seg_n[q];
-        for (uint32_t i = lane; i < cnt; i += kWave) err |= scatter_event<WIDE, CODE_HERE, CODE2>(S, q, i, T);
+        for (uint32_t i = lane; i < cnt; i += kWave) {
+            uint32_t x;
+            E e;
+            err |= make_event<WIDE, CODE_HERE, CODE2>(S, q, i, T, x, e);
+            const uint32_t pos = atomicAdd(&base[x >> S.pshift], 1u);
+            S.pgk[pos] = e.gk;
+            S.pw[pos] = e.w;
+            S.ptl[pos] = (uint16_t)(x & mask);
+            if constexpr (WIDE) {
+                S.ptpos[pos] = e.tpos;
+                S.pslot[pos] = e.slot;
+            }
+        }
     }
     if (err) atomicOr(&S.ctl[1], err);
 }
 
+constexpr uint32_t kStageBytes = 32768;
+
+// One workgroup per target partition: counting sort of the partition's events by target in LDS
+// (global scratch when they do not fit), then one thread per target folds its events.
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
-__global__ void __launch_bounds__(kBlock) k_fold(Step S) {
+__global__ void __launch_bounds__(kBlock) k_pfold(Step S) {
+    typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
+    constexpr uint32_t kCap = kStageBytes / sizeof(E);
+    __shared__ E lds_stage[kCap];
+    extern __shared__ uint32_t sh[];
+    const uint32_t PS = 1u << S.pshift;
+    uint32_t* cnt = sh;
+    uint32_t* off = sh + PS;
+    uint32_t* fill = sh + 2 * PS;
+    const uint32_t p = blockIdx.x;
     const uint32_t T = S.ctl[0];
+    const uint32_t start = S.hist[(uint64_t)p * S.nwg];
+    const uint32_t end = p + 1 < S.nparts ? S.hist[(uint64_t)(p + 1) * S.nwg] : S.seg_count[1];
+    const uint32_t c = end - start;
+    for (uint32_t t = threadIdx.x; t < PS; t += blockDim.x) cnt[t] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) atomicAdd(&cnt[S.ptl[start + i]], 1u);
+    __syncthreads();
+    uint32_t run = 0;
+    for (uint32_t t0 = 0; t0 < PS; t0 += blockDim.x) {
+        const uint32_t v = cnt[t0 + threadIdx.x];
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, &tot);
+        off[t0 + threadIdx.x] = run + ex;
+        fill[t0 + threadIdx.x] = run + ex;
+        run += tot;
+    }
+    __syncthreads();
+    E* st = c <= kCap ? lds_stage : (E*)S.stage + start;
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) {
+        const uint32_t pos = atomicAdd(&fill[S.ptl[start + i]], 1u);
+        E e;
+        e.gk = S.pgk[start + i];
+        e.w = S.pw[start + i];
+        if constexpr (WIDE) {
+            e.tpos = S.ptpos[start + i];
+            e.slot = S.pslot[start + i];
+        }
+        st[pos] = e;
+    }
+    __syncthreads();
     uint32_t err = 0;
-    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < S.n; x += gridDim.x * blockDim.x)
-        err |= fold_neuron<WIDE, STDP, CODE_HERE, CODE2>(S, x, T);
+    for (uint32_t t = threadIdx.x; t < PS; t += blockDim.x) {
+        const uint32_t x = (p << S.pshift) + t;
+        if (x < S.n && cnt[t]) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2>(S, st + off[t], cnt[t], x, T);
+    }
     if (err) atomicOr(&S.ctl[1], err);
 }
 
@@ -320,7 +391,7 @@ __global__ void k_advance(Step S, uint32_t t0) {
 }
 
 // readback gathers (connectome order)
-__global__ void k_gather_weights(const uint64_t* slot_of, const float* w, const uint32_t* tpos, const StdpRec* stdp,
+__global__ void k_gather_weights(const uint32_t* slot_of, const float* w, const uint32_t* tpos, const StdpRec* stdp,
                                  uint64_t e, float* out) {
     for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < e; j += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t s = slot_of[j];
@@ -330,7 +401,7 @@ __global__ void k_gather_weights(const uint64_t* slot_of, const float* w, const 
 }
 
 template <bool CODE2>
-__global__ void k_gather_codes(const uint64_t* slot_of, const CodeRec2* c2, const CodeRecN* cn, uint64_t e,
+__global__ void k_gather_codes(const uint32_t* slot_of, const CodeRec2* c2, const CodeRecN* cn, uint64_t e,
                                uint32_t* match, uint32_t* learned) {
     for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < e; j += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t s = slot_of[j];
@@ -345,10 +416,189 @@ __global__ void k_gather_codes(const uint64_t* slot_of, const CodeRec2* c2, cons
     }
 }
 
-__global__ void k_gather_u32(const uint64_t* slot_of, const uint32_t* src, uint64_t e, uint32_t* out) {
+// last_pre / last_post of every synapse (SynapseRuntimeState, reference
+// include/snncuda/runtime/SynapseDendriteProcessor.h:25-26): plastic synapses keep them in their
+// STDP record; for the others last_post is the target's last fire tick (every fire stamps all
+// incoming synapses, NetworkPropagator.cpp:95-102) and last_pre is the last arrival tick minus the
+// delay (SynapseDendriteProcessor.cpp:95,125).
+__global__ void k_gather_ticks(const uint32_t* slot_of, const uint32_t* tpos, const StdpRec* stdp, const CodeRec2* c2,
+                               const CodeRecN* cn, const uint16_t* delay, const uint32_t* tgt,
+                               const uint32_t* last_fired, uint64_t e, int64_t* last_pre, int64_t* last_post) {
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < e; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t s = slot_of[j];
+        const uint32_t t = tpos ? tpos[s] : kNone;
+        uint32_t pre, post;
+        if (t != kNone) {
+            pre = stdp[t].last_pre;
+            post = stdp[t].last_post;
+        } else {
+            const uint32_t a = c2 ? code_last_arrival2(c2[s]) : code_last_arrival(cn[s]);
+            const uint32_t dl = delay[s];
+            pre = a == kNone ? kNone : (a >= dl ? a - dl : a);
+            post = last_fired[tgt[s]];
+        }
+        last_pre[j] = pre == kNone ? -1 : (int64_t)pre;
+        last_post[j] = post == kNone ? -1 : (int64_t)post;
+    }
+}
+
+__global__ void k_gather_u32(const uint32_t* slot_of, const uint32_t* src, uint64_t e, uint32_t* out) {
     for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < e; j += (uint64_t)gridDim.x * blockDim.x)
         out[j] = src[slot_of[j]];
 }
+
+// ------------------------------------------------------------------------------------ synthetic E/I networks
+// Device generator for the seeded E/I network family of snncuda_amd/workloads.py (ei_network with
+// one code per synapse): every draw is the same counter hash, so a generated network is identical
+// to the host-built one. Connectome order is source-major (edge j of source s has index
+// row[s] + j); slots are sorted by (source, delay, j) exactly as build_host_net sorts them.
+
+struct GenParams {
+    uint64_t n, nall, n_exc;
+    float w_exc, w_inh;
+    uint32_t delay_max, n_comps;
+    uint32_t comps[4];
+    bool plastic;
+    uint64_t stream_key[8];
+    const uint64_t* row;
+    uint32_t* grp;
+    uint32_t* tgt;
+    float* w;
+    uint8_t* meta;
+    uint16_t* delay;
+    uint32_t* slot_of;
+    uint32_t* incnt;
+    const uint64_t* in_ptr;
+    uint32_t* tpos;
+    StdpRec* stdp;
+    uint32_t* tpos_conn;
+};
+
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+constexpr uint64_t kGold = 0x9E3779B97F4A7C15ull;
+__device__ __forceinline__ uint64_t ghash(uint64_t key, uint64_t idx) { return mix64(key + idx * kGold); }
+__device__ __forceinline__ uint32_t gbelow(uint64_t h, uint64_t n) { return (uint32_t)(((h >> 32) * n) >> 32); }
+__device__ __forceinline__ float gunif24(uint64_t h) { return (float)(uint32_t)(h >> 40) * 0x1p-24f; }
+
+struct GenEdge {
+    uint32_t tgt, delay;
+    float w;
+    uint8_t meta;
+};
+
+__device__ __forceinline__ uint32_t gen_delay(const GenParams& P, uint64_t s, uint64_t ctr) {
+    return 1 + gbelow(ghash(P.stream_key[s < P.n ? 5 : 7], ctr), P.delay_max);
+}
+
+__device__ __forceinline__ uint32_t gen_target(const GenParams& P, uint64_t s, uint64_t ctr) {
+    if (s < P.n) {
+        uint32_t t = gbelow(ghash(P.stream_key[1], ctr), P.n - 1);
+        return t + (t >= s ? 1u : 0u);
+    }
+    return gbelow(ghash(P.stream_key[2], ctr), P.n);
+}
+
+__device__ GenEdge gen_edge(const GenParams& P, uint64_t s, uint64_t j) {
+    const uint64_t ctr = (s << 24) + j;
+    GenEdge g;
+    g.tgt = gen_target(P, s, ctr);
+    g.delay = gen_delay(P, s, ctr);
+    uint32_t comp, rec;
+    if (s < P.n) {
+        const bool exc = s < P.n_exc;
+        const float u = gunif24(ghash(P.stream_key[3], ctr));
+        g.w = exc ? u * P.w_exc : u * P.w_inh;
+        rec = exc ? kAmpa : kGabaA;
+        comp = P.comps[gbelow(ghash(P.stream_key[4], ctr), P.n_comps)];
+    } else {
+        g.w = gunif24(ghash(P.stream_key[6], ctr)) * 0.5f;
+        rec = kAmpa;
+        comp = kBasal;
+    }
+    g.meta = (uint8_t)(rec | (comp << 2) | ((P.plastic ? 1u : 0u) << 4));
+    return g;
+}
+
+// pass 1: per-source delay histogram -> group table; in-degree counts of plastic synapses
+__global__ void __launch_bounds__(kBlock) k_gen_pass1(GenParams P) {
+    __shared__ uint32_t hist[kBlock / kWave][64];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wv = threadIdx.x / kWave;
+    const uint64_t s = (uint64_t)blockIdx.x * (kBlock / kWave) + wv;
+    const bool valid = s < P.nall;
+    hist[wv][lane] = 0;
+    __syncthreads();
+    const uint64_t r0 = valid ? P.row[s] : 0, len = valid ? P.row[s + 1] - r0 : 0;
+    for (uint64_t j = lane; j < len; j += kWave) {
+        const uint64_t ctr = (s << 24) + j;
+        atomicAdd(&hist[wv][gen_delay(P, s, ctr)], 1u);
+        if (P.plastic) atomicAdd(&P.incnt[gen_target(P, s, ctr)], 1u);
+    }
+    __syncthreads();
+    if (valid) {
+        // lane L-1 holds the count of lag L; exclusive scan gives the group starts
+        const uint32_t c = (lane + 1 <= P.delay_max) ? hist[wv][lane + 1] : 0u;
+        const uint32_t incl = wave_incl_scan(c);
+        const uint32_t W = P.delay_max + 1;
+        if (lane < P.delay_max) P.grp[s * W + lane] = incl - c;
+        if (lane == 0) P.grp[s * W + P.delay_max] = (uint32_t)len;
+    }
+}
+
+// pass 2: stable placement of each source's synapses by delay, plus target-major STDP records
+__global__ void __launch_bounds__(kBlock) k_gen_pass2(GenParams P) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t s = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (s >= P.nall) return;  // whole wave; no block barriers below
+    const uint32_t W = P.delay_max + 1;
+    const uint64_t r0 = P.row[s], len = P.row[s + 1] - r0;
+    uint32_t running = lane < P.delay_max ? P.grp[s * W + lane] : 0u;  // next free position of lag lane+1
+    const uint64_t lt = (1ull << lane) - 1;
+    for (uint64_t c = 0; c < len; c += kWave) {
+        const uint64_t j = c + lane;
+        const bool ok = j < len;
+        GenEdge g{};
+        if (ok) g = gen_edge(P, s, j);
+        uint32_t pos = 0;
+        for (uint32_t L = 1; L <= P.delay_max; ++L) {
+            const uint64_t m = __ballot(ok && g.delay == L);
+            if (!m) continue;
+            const uint32_t base = __shfl(running, (int)(L - 1), kWave);
+            if (ok && g.delay == L) pos = base + (uint32_t)__popcll(m & lt);
+            if (lane == L - 1) running += (uint32_t)__popcll(m);
+        }
+        if (ok) {
+            const uint64_t slot = r0 + pos;
+            P.tgt[slot] = g.tgt;
+            P.w[slot] = g.w;
+            P.meta[slot] = g.meta;
+            P.delay[slot] = (uint16_t)g.delay;
+            P.slot_of[r0 + j] = (uint32_t)slot;
+            if (P.plastic) {
+                const uint64_t tp = P.in_ptr[g.tgt] + atomicAdd(&P.incnt[g.tgt], 1u);
+                P.tpos[slot] = (uint32_t)tp;
+                StdpRec r;
+                r.weight = g.w;
+                r.last_pre = kNone;
+                r.last_post = kNone;
+                r.aux = 0;  // wmax table entry 0 = 2.0
+                P.stdp[tp] = r;
+                P.tpos_conn[tp] = (uint32_t)(r0 + j);
+            }
+        }
+    }
+}
+
+__global__ void k_widen(const uint32_t* in, uint64_t n, uint64_t* out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+uint64_t stream_key(uint64_t seed, uint64_t stream) { return mix64(seed * kGold + stream); }
 
 // ------------------------------------------------------------------------------------ host side
 
@@ -383,7 +633,7 @@ struct PendingInject {
 };
 
 enum KernelId { KPlan, KSegments, KScanSeg, KCount, KScanTgt, KScatter, KFold, KScanFire, KFireWrite, KPost, KAdvance, KCount_ };
-const char* kKernelNames[KCount_] = {"plan", "segments", "scan_segments", "count", "scan_targets", "scatter",
+const char* kKernelNames[KCount_] = {"plan", "segments", "scan_segments", "count", "scan_partitions", "scatter",
                                      "fold", "scan_fires", "fire_write", "post", "advance"};
 
 }  // namespace
@@ -409,8 +659,8 @@ struct snn_net {
     uint32_t* stat_buf = nullptr;
     uint64_t stat_cap = 0;
     // readback helpers
-    uint64_t* slot_of = nullptr;
-    uint64_t* tpos_conn_dev = nullptr;
+    uint32_t* slot_of = nullptr;
+    uint32_t* tpos_conn_dev = nullptr;
     uint64_t n_tpos = 0;
     // profiling
     bool profiling = false;
@@ -439,12 +689,14 @@ void launch_scan(snn_net* net, Src src, uint32_t* out, uint32_t* total, uint64_t
 
 template <bool WIDE, bool CODE_HERE, bool CODE2>
 void launch_scatter(snn_net* net) {
-    hipLaunchKernelGGL((k_scatter<WIDE, CODE_HERE, CODE2>), dim3(net->grid), dim3(kBlock), 0, net->stream, net->S);
+    hipLaunchKernelGGL((k_pscatter<WIDE, CODE_HERE, CODE2>), dim3(net->S.nwg), dim3(kBlock), net->S.nparts * 4,
+                       net->stream, net->S);
 }
 
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
 void launch_fold(snn_net* net) {
-    hipLaunchKernelGGL((k_fold<WIDE, STDP, CODE_HERE, CODE2>), dim3(net->fold_grid), dim3(kBlock), 0, net->stream, net->S);
+    hipLaunchKernelGGL((k_pfold<WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kBlock),
+                       3u * (1u << net->S.pshift) * 4, net->stream, net->S);
 }
 
 void dispatch_scatter(snn_net* net) {
@@ -521,10 +773,10 @@ void launch_tick(snn_net* net, uint32_t t0, Timer& tm) {
                 std::min<uint64_t>(S.ev_cap, kMaxEventsPerTick), kErrEvents, &S.seg_count[0]);
     tm.end();
     tm.begin(KCount);
-    hipLaunchKernelGGL(k_count, dim3(net->grid), dim3(kBlock), 0, st, S);
+    hipLaunchKernelGGL(k_pcount, dim3(S.nwg), dim3(kBlock), S.nparts * 4, st, S);
     tm.end();
     tm.begin(KScanTgt);
-    launch_scan(net, PlainSrc{S.tcnt, nullptr, S.n}, S.toff, nullptr, 0, 0, nullptr);
+    launch_scan(net, PlainSrc{S.hist, nullptr, S.nparts * S.nwg}, S.hist, nullptr, 0, 0, nullptr);
     tm.end();
     tm.begin(KScatter);
     dispatch_scatter(net);
@@ -593,6 +845,103 @@ void host_stdp_tables(std::vector<float>& ltp, std::vector<float>& ltd) {
     }
 }
 
+
+int check_device(const snn_options& o) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SNN_ENODEVICE, "no HIP device");
+    if (o.device < 0 || o.device >= ndev) return fail(SNN_EINVAL, "device ordinal out of range");
+    return 0;
+}
+
+// device, stream, modes and per-neuron state
+void open_net(snn_net* net, const snn_options& o, uint32_t n, uint64_t e, uint32_t lmax, bool plastic, bool ve_identity,
+              bool code_compact) {
+    net->device = o.device;
+    check(hipSetDevice(o.device), "hipSetDevice");
+    check(hipStreamCreate(&net->stream), "hipStreamCreate");
+    net->n = n;
+    net->e = e;
+    net->stdp = plastic;
+    net->code_here = ve_identity;
+    net->wide = plastic || !ve_identity;
+    net->code2 = code_compact;
+    auto& own = net->owned;
+    Step& S = net->S;
+    S.n = n;
+    S.lmax = lmax;
+    if (code_compact) S.code2 = dev_alloc<CodeRec2>(own, e);
+    else S.codeN = dev_alloc<CodeRecN>(own, e);
+    S.pre_count = o.track_pre_count ? dev_alloc<uint32_t>(own, e) : nullptr;
+    S.dyn = dev_alloc<NeuronDyn>(own, n);
+    S.th = dev_alloc<TemporalHdr>(own, n);
+    S.cand = dev_alloc<Runs>(own, n);
+    S.refs = dev_alloc<TemporalRefs>(own, n);
+    S.last_fired = dev_alloc<uint32_t>(own, n, false);
+    check(hipMemset(S.last_fired, 0xFF, (size_t)n * 4), "hipMemset");
+    S.decay_tab = dev_upload(own, host_decay_table());
+    std::vector<float> ltp, ltd;
+    host_stdp_tables(ltp, ltd);
+    S.ltp = dev_upload(own, ltp);
+    S.ltd = dev_upload(own, ltd);
+}
+
+// capacities and per-tick work buffers
+void finish_net(snn_net* net, const snn_options& o, uint64_t nve) {
+    auto& own = net->owned;
+    Step& S = net->S;
+    const uint32_t n = S.n;
+    const uint64_t fire_cap = o.fire_log_capacity ? o.fire_log_capacity : std::max<uint64_t>(1u << 16, 8ull * n);
+    uint64_t ev_cap = o.max_events_per_tick ? o.max_events_per_tick
+                                            : std::max<uint64_t>(1u << 20, std::min<uint64_t>(2 * nve, 1ull << 26));
+    ev_cap = std::min<uint64_t>(ev_cap, kMaxEventsPerTick);
+    ev_cap = (ev_cap + 31) & ~31ull;
+    const uint64_t seg_cap = std::min<uint64_t>(fire_cap + S.lmax + 1, 0xFFFFFFF0ull);
+    S.fire_cap = fire_cap;
+    S.firelog = dev_alloc<uint32_t>(own, fire_cap);
+    S.firemult = dev_alloc<uint32_t>(own, fire_cap);
+    S.tick_fstart = dev_alloc<uint64_t>(own, S.lmax + 1);
+    S.tick_fcount = dev_alloc<uint32_t>(own, S.lmax + 1);
+    S.groups = dev_alloc<Group>(own, 2 * (size_t)S.lmax + 1);
+    S.seg_count = dev_alloc<uint32_t>(own, 4);
+    S.seg_cap = seg_cap;
+    S.ev_cap = ev_cap;
+    S.seg_v0 = dev_alloc<uint64_t>(own, seg_cap, false);
+    S.seg_n = dev_alloc<uint32_t>(own, seg_cap, false);
+    S.seg_off = dev_alloc<uint32_t>(own, seg_cap, false);
+    S.seg_mult = dev_alloc<uint32_t>(own, seg_cap, false);
+    // target partitions of 256 neurons (more when the partition count would exceed 16384)
+    uint32_t pshift = 8;
+    while (((uint64_t)n + (1ull << pshift) - 1) >> pshift > 16384) ++pshift;
+    S.pshift = pshift;
+    S.nparts = (uint32_t)(((uint64_t)n + (1ull << pshift) - 1) >> pshift);
+    int ncu = 256;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, o.device) == hipSuccess) ncu = pr.multiProcessorCount;
+    S.nwg = (uint32_t)std::max(1, ncu);
+    S.hist = dev_alloc<uint32_t>(own, (uint64_t)S.nparts * S.nwg);
+    S.pgk = dev_alloc<uint32_t>(own, ev_cap, false);
+    S.pw = dev_alloc<float>(own, ev_cap, false);
+    S.ptl = dev_alloc<uint16_t>(own, ev_cap, false);
+    S.ptpos = net->wide ? dev_alloc<uint32_t>(own, ev_cap, false) : nullptr;
+    S.pslot = net->wide ? dev_alloc<uint32_t>(own, ev_cap, false) : nullptr;
+    S.stage = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
+    S.fire_bits = dev_alloc<uint32_t>(own, ev_cap / 32);
+    S.fire_neuron = dev_alloc<uint32_t>(own, ev_cap, false);
+    S.fire_mult = dev_alloc<uint32_t>(own, ev_cap, false);
+    S.fire_word_off = dev_alloc<uint32_t>(own, ev_cap / 32, false);
+    S.ctl = dev_alloc<uint32_t>(own, 4);
+    S.totals = dev_alloc<uint64_t>(own, 4);
+    S.raster_cap = o.raster_capacity ? o.raster_capacity : (1ull << 22);
+    S.raster = dev_alloc<uint32_t>(own, S.raster_cap, false);
+    if (!net->partials) net->partials = dev_alloc<uint32_t>(own, kScanBlocks);
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) cus = prop.multiProcessorCount;
+    net->grid = (uint32_t)std::max(64, cus * 4);
+    net->fold_grid = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock, 65535ull * 4);
+    check(hipDeviceSynchronize(), "create");
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------ C ABI
@@ -614,12 +963,10 @@ int snn_net_create(const snn_net_desc* desc, const snn_options* opts, snn_net** 
     if (!desc || !out) return fail(SNN_EINVAL, "null argument");
     *out = nullptr;
     return guarded([&]() -> int {
-        int ndev = 0;
-        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SNN_ENODEVICE, "no HIP device");
         snn_options o{};
         if (opts) o = *opts;
-        if (o.device < 0 || o.device >= ndev) return fail(SNN_EINVAL, "device ordinal out of range");
-
+        const int rc = check_device(o);
+        if (rc) return rc;
         NetDesc d;
         d.n_neurons = desc->n_neurons;
         d.threshold = desc->threshold;
@@ -645,31 +992,18 @@ int snn_net_create(const snn_net_desc* desc, const snn_options* opts, snn_net** 
         HostNet h = build_host_net(d);
 
         std::unique_ptr<snn_net> net(new snn_net());
-        net->device = o.device;
-        check(hipSetDevice(o.device), "hipSetDevice");
-        check(hipStreamCreate(&net->stream), "hipStreamCreate");
-        net->n = h.n;
-        net->e = h.e;
-        net->stdp = h.any_plastic;
-        net->code_here = h.ve_identity;
-        net->wide = h.any_plastic || !h.ve_identity;
-        net->code2 = h.code_compact;
+        open_net(net.get(), o, h.n, h.e, h.lmax, h.any_plastic, h.ve_identity, h.code_compact);
         auto& own = net->owned;
         Step& S = net->S;
-        S.n = h.n;
-        S.lmax = h.lmax;
         S.row = dev_upload(own, h.row);
         S.grp = dev_upload(own, h.grp);
         S.ve = h.ve_identity ? nullptr : dev_upload(own, h.ve);
         S.tgt = dev_upload(own, h.tgt);
         S.w = dev_upload(own, h.w);
         S.meta = dev_upload(own, h.meta);
-        S.delay = h.any_plastic ? dev_upload(own, h.delay) : nullptr;
+        S.delay = dev_upload(own, h.delay);
         S.expected = h.expected.empty() ? nullptr : dev_upload(own, h.expected);
         S.expected_all = h.expected_all ? h.expected_all : 2;
-        if (h.code_compact) S.code2 = dev_alloc<CodeRec2>(own, h.e);
-        else S.codeN = dev_alloc<CodeRecN>(own, h.e);
-        S.pre_count = o.track_pre_count ? dev_alloc<uint32_t>(own, h.e) : nullptr;
         if (h.any_plastic) {
             S.tpos = dev_upload(own, h.tpos);
             S.in_ptr = dev_upload(own, h.in_ptr);
@@ -680,62 +1014,122 @@ int snn_net_create(const snn_net_desc* desc, const snn_options* opts, snn_net** 
             net->n_tpos = h.stdp.size();
         }
         S.np = dev_upload(own, h.params);
-        S.dyn = dev_alloc<NeuronDyn>(own, h.n);
-        S.th = dev_alloc<TemporalHdr>(own, h.n);
-        S.cand = dev_alloc<Runs>(own, h.n);
-        S.refs = dev_alloc<TemporalRefs>(own, h.n);
-        {
-            std::vector<uint32_t> lf(h.n, kNone);
-            S.last_fired = dev_upload(own, lf);
-        }
-        S.decay_tab = dev_upload(own, host_decay_table());
-        {
-            std::vector<float> ltp, ltd;
-            host_stdp_tables(ltp, ltd);
-            S.ltp = dev_upload(own, ltp);
-            S.ltd = dev_upload(own, ltd);
-        }
         net->slot_of = dev_upload(own, h.slot_of);
+        finish_net(net.get(), o, h.nve);
+        *out = net.release();
+        return SNN_OK;
+    });
+}
 
-        // capacities
-        const uint64_t fire_cap = o.fire_log_capacity ? o.fire_log_capacity : std::max<uint64_t>(1u << 16, 8ull * h.n);
-        uint64_t ev_cap = o.max_events_per_tick ? o.max_events_per_tick
-                                                : std::max<uint64_t>(1u << 20, std::min<uint64_t>(2 * h.nve, 1ull << 26));
-        ev_cap = std::min<uint64_t>(ev_cap, kMaxEventsPerTick);
-        ev_cap = (ev_cap + 31) & ~31ull;
-        const uint64_t seg_cap = std::min<uint64_t>(fire_cap + h.lmax + 1, 0xFFFFFFF0ull);
-        S.fire_cap = fire_cap;
-        S.firelog = dev_alloc<uint32_t>(own, fire_cap);
-        S.firemult = dev_alloc<uint32_t>(own, fire_cap);
-        S.tick_fstart = dev_alloc<uint64_t>(own, h.lmax + 1);
-        S.tick_fcount = dev_alloc<uint32_t>(own, h.lmax + 1);
-        S.groups = dev_alloc<Group>(own, 2 * (size_t)h.lmax + 1);
-        S.seg_count = dev_alloc<uint32_t>(own, 4);
-        S.seg_cap = seg_cap;
-        S.ev_cap = ev_cap;
-        S.seg_v0 = dev_alloc<uint64_t>(own, seg_cap, false);
-        S.seg_n = dev_alloc<uint32_t>(own, seg_cap, false);
-        S.seg_off = dev_alloc<uint32_t>(own, seg_cap, false);
-        S.seg_mult = dev_alloc<uint32_t>(own, seg_cap, false);
-        S.tcnt = dev_alloc<uint32_t>(own, h.n);
-        S.toff = dev_alloc<uint32_t>(own, h.n, false);
-        S.rank = dev_alloc<uint32_t>(own, ev_cap, false);
-        S.bins = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
-        S.fire_bits = dev_alloc<uint32_t>(own, ev_cap / 32);
-        S.fire_neuron = dev_alloc<uint32_t>(own, ev_cap, false);
-        S.fire_mult = dev_alloc<uint32_t>(own, ev_cap, false);
-        S.fire_word_off = dev_alloc<uint32_t>(own, ev_cap / 32, false);
-        S.ctl = dev_alloc<uint32_t>(own, 4);
-        S.totals = dev_alloc<uint64_t>(own, 4);
-        S.raster_cap = o.raster_capacity ? o.raster_capacity : (1ull << 22);
-        S.raster = dev_alloc<uint32_t>(own, S.raster_cap, false);
-        net->partials = dev_alloc<uint32_t>(own, kScanBlocks);
-        int cus = 256;
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) cus = prop.multiProcessorCount;
-        net->grid = (uint32_t)std::max(64, cus * 4);
-        net->fold_grid = (uint32_t)std::min<uint64_t>((h.n + kBlock - 1) / kBlock, 65535ull * 4);
-        check(hipDeviceSynchronize(), "create");
+int snn_net_create_ei(const snn_ei_desc* g, const snn_options* opts, snn_net** out) {
+    if (!g || !out) return fail(SNN_EINVAL, "null argument");
+    *out = nullptr;
+    return guarded([&]() -> int {
+        snn_options o{};
+        if (opts) o = *opts;
+        const int rc = check_device(o);
+        if (rc) return rc;
+        if (g->n < 2 || !g->fanout) return fail(SNN_EINVAL, "need at least two network neurons and a fan-out array");
+        if (g->delay_max < 1 || g->delay_max > 63) return fail(SNN_EINVAL, "delay_max must be in [1, 63]");
+        if (g->n_comps < 1 || g->n_comps > 4) return fail(SNN_EINVAL, "n_comps must be in [1, 4]");
+        if (g->window > 65535 || g->max_refs < 1) return fail(SNN_EINVAL, "bad temporal parameters");
+        const uint64_t nall = g->n + g->n_sources;
+        if (nall >= 0xFFFFFFFFull) return fail(SNN_EINVAL, "too many neurons");
+        std::vector<uint64_t> row(nall + 1, 0);
+        for (uint64_t i = 0; i < g->n; ++i) {
+            if (g->fanout[i] >= (1u << 24)) return fail(SNN_EINVAL, "fan-out must be below 2^24");
+            row[i + 1] = row[i] + g->fanout[i];
+        }
+        if (g->source_fanout >= (1u << 24)) return fail(SNN_EINVAL, "fan-out must be below 2^24");
+        for (uint64_t i = g->n; i < nall; ++i) row[i + 1] = row[i] + g->source_fanout;
+        const uint64_t e = row[nall];
+        if (e >= 0xFFFFFFFFull) return fail(SNN_EINVAL, "more than 2^32-2 synapses on one device");
+        const uint32_t lmax = g->delay_max;
+
+        std::unique_ptr<snn_net> net(new snn_net());
+        const bool plastic = g->plastic != 0;
+        const bool compact = g->max_refs == 1;
+        open_net(net.get(), o, (uint32_t)nall, e, lmax, plastic, true, compact);
+        auto& own = net->owned;
+        Step& S = net->S;
+        hipStream_t st = net->stream;
+        S.row = dev_upload(own, row);
+        uint32_t* grp = dev_alloc<uint32_t>(own, nall * (lmax + 1));
+        S.grp = grp;
+        S.ve = nullptr;
+        uint32_t* tgt = dev_alloc<uint32_t>(own, e, false);
+        float* w = dev_alloc<float>(own, e, false);
+        uint8_t* meta = dev_alloc<uint8_t>(own, e, false);
+        uint16_t* delay = dev_alloc<uint16_t>(own, e, false);
+        S.tgt = tgt;
+        S.w = w;
+        S.meta = meta;
+        S.delay = delay;
+        S.expected = nullptr;
+        S.expected_all = 2;
+        net->slot_of = dev_alloc<uint32_t>(own, e, false);
+
+        GenParams gp{};
+        gp.n = g->n;
+        gp.nall = nall;
+        gp.n_exc = g->n_exc;
+        gp.w_exc = g->w_exc;
+        gp.w_inh = g->w_inh;
+        gp.delay_max = g->delay_max;
+        gp.n_comps = g->n_comps;
+        for (int k = 0; k < 4; ++k) gp.comps[k] = g->comps[k];
+        gp.plastic = plastic;
+        for (uint32_t k = 0; k < 8; ++k) gp.stream_key[k] = stream_key(g->seed, k);
+        gp.row = S.row;
+        gp.grp = grp;
+        gp.tgt = tgt;
+        gp.w = w;
+        gp.meta = meta;
+        gp.delay = delay;
+        gp.slot_of = net->slot_of;
+        uint32_t* incnt = nullptr;
+        if (plastic) {
+            incnt = dev_alloc<uint32_t>(own, nall);
+            gp.incnt = incnt;
+        }
+        const uint32_t rows_per_block = kBlock / kWave;
+        const dim3 ggrid((uint32_t)((nall + rows_per_block - 1) / rows_per_block));
+        hipLaunchKernelGGL(k_gen_pass1, ggrid, dim3(kBlock), 0, st, gp);
+        if (plastic) {
+            // target-major STDP records
+            uint32_t* in32 = dev_alloc<uint32_t>(own, nall + 1);
+            net->partials = dev_alloc<uint32_t>(own, kScanBlocks);
+            launch_scan(net.get(), PlainSrc{incnt, nullptr, (uint32_t)nall}, in32, &in32[nall], 0, 0, nullptr);
+            uint64_t* in_ptr = dev_alloc<uint64_t>(own, nall + 1);
+            hipLaunchKernelGGL(k_widen, dim3(1024), dim3(kBlock), 0, st, in32, nall + 1, in_ptr);
+            check(hipMemsetAsync(incnt, 0, nall * 4, st), "memset");
+            S.in_ptr = in_ptr;
+            uint32_t* tpos = dev_alloc<uint32_t>(own, e, false);
+            StdpRec* stdp = dev_alloc<StdpRec>(own, e, false);
+            uint32_t* tconn = dev_alloc<uint32_t>(own, e, false);
+            S.tpos = tpos;
+            S.stdp = stdp;
+            S.upd_count = dev_alloc<uint32_t>(own, e);
+            std::vector<float> wm(1, 2.0f);  // max_weight 2.0 on every synapse: min(2, 2)
+            S.wmax = dev_upload(own, wm);
+            net->tpos_conn_dev = tconn;
+            net->n_tpos = e;
+            gp.in_ptr = in_ptr;
+            gp.tpos = tpos;
+            gp.stdp = stdp;
+            gp.tpos_conn = tconn;
+        }
+        hipLaunchKernelGGL(k_gen_pass2, ggrid, dim3(kBlock), 0, st, gp);
+        std::vector<NeuronParams> params(nall);
+        for (uint64_t i = 0; i < nall; ++i) {
+            params[i].threshold = i < g->n ? g->threshold : g->source_threshold;
+            params[i].window = g->window;
+            params[i].sim_thr = g->sim_thr;
+            params[i].max_refs = g->max_refs;
+        }
+        S.np = dev_upload(own, params);
+        check(hipStreamSynchronize(st), "generate");
+        finish_net(net.get(), o, e);
         *out = net.release();
         return SNN_OK;
     });
@@ -948,24 +1342,23 @@ int snn_net_read_synapses(snn_net* net, float* weight, uint64_t* pre_count, uint
             check(hipStreamSynchronize(net->stream), "sync");
             for (uint64_t j = 0; j < e; ++j) pre_count[j] = h[j];
         }
-        if (update_count || last_pre || last_post) {
-            if (update_count) std::fill(update_count, update_count + e, 0);
-            if (last_pre) std::fill(last_pre, last_pre + e, -1);
-            if (last_post) std::fill(last_post, last_post + e, -1);
+        if (last_pre || last_post) {
+            int64_t* dpre = dev_alloc<int64_t>(tmp, e, false);
+            int64_t* dpost = dev_alloc<int64_t>(tmp, e, false);
+            hipLaunchKernelGGL(k_gather_ticks, g, b, 0, net->stream, net->slot_of, net->S.tpos, net->S.stdp, net->S.code2,
+                               net->S.codeN, net->S.delay, net->S.tgt, net->S.last_fired, e, dpre, dpost);
+            if (last_pre) check(hipMemcpyAsync(last_pre, dpre, e * 8, hipMemcpyDeviceToHost, net->stream), "read ticks");
+            if (last_post) check(hipMemcpyAsync(last_post, dpost, e * 8, hipMemcpyDeviceToHost, net->stream), "read ticks");
+        }
+        if (update_count) {
+            std::fill(update_count, update_count + e, 0);
             if (net->n_tpos) {
                 const uint64_t nt = net->n_tpos;
-                std::vector<StdpRec> rec(nt);
                 std::vector<uint32_t> upd(nt);
-                std::vector<uint64_t> conn(nt);
-                check(hipMemcpy(rec.data(), net->S.stdp, nt * sizeof(StdpRec), hipMemcpyDeviceToHost), "read stdp");
+                std::vector<uint32_t> conn(nt);
                 check(hipMemcpy(upd.data(), net->S.upd_count, nt * 4, hipMemcpyDeviceToHost), "read stdp");
-                check(hipMemcpy(conn.data(), net->tpos_conn_dev, nt * 8, hipMemcpyDeviceToHost), "read stdp");
-                for (uint64_t t = 0; t < nt; ++t) {
-                    const uint64_t j = conn[t];
-                    if (update_count) update_count[j] = upd[t];
-                    if (last_pre) last_pre[j] = rec[t].last_pre == kNone ? -1 : (int64_t)rec[t].last_pre;
-                    if (last_post) last_post[j] = rec[t].last_post == kNone ? -1 : (int64_t)rec[t].last_post;
-                }
+                check(hipMemcpy(conn.data(), net->tpos_conn_dev, nt * 4, hipMemcpyDeviceToHost), "read stdp");
+                for (uint64_t t = 0; t < nt; ++t) update_count[conn[t]] = upd[t];
             }
         }
         check(hipStreamSynchronize(net->stream), "read synapses");

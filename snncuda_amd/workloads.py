@@ -171,6 +171,9 @@ def ei_network(n: int, fanout, seed: int, *, n_sources: int, source_fanout: int,
     """
     n_exc = int(round(n * exc_frac))
     fan = np.broadcast_to(np.asarray(fanout, dtype=np.int64), (n,)).copy()
+    spec = dict(n=n, n_sources=n_sources, fanout=fan, source_fanout=source_fanout, seed=seed, n_exc=n_exc,
+                w_exc=w_exc, w_inh=w_inh, delay_max=delay_max, compartments=tuple(compartments), plastic=plastic,
+                threshold=threshold, window=window, sim_thr=sim_thr, max_refs=max_refs)
     src, tgt, ctr = _fanout_edges(seed, 1, np.arange(n), fan, n, 0, True)
     ssrc, stgt, sctr = _fanout_edges(seed, 2, np.arange(n, n + n_sources), np.full(n_sources, source_fanout), n, 0, False)
     e_net = src.shape[0]
@@ -207,7 +210,7 @@ def ei_network(n: int, fanout, seed: int, *, n_sources: int, source_fanout: int,
         np.concatenate([receptor, np.full(ssrc.shape[0], AMPA, np.uint8)]),
         np.full(e, 1 if plastic else 0, np.uint8), code_ptr=code_ptr, codes=codes, name=name,
         info=dict(n=n, n_sources=n_sources, e_net=int(e_net), seed=seed, ticks=ticks,
-                  threshold=threshold, p_source=p_source),
+                  threshold=threshold, p_source=p_source, spec=spec if n_codes <= 1 and code_max == 0 else None),
     )
     net.commands = poisson_commands(seed, n, n_sources, p_source, ticks)
     return net
@@ -340,3 +343,21 @@ def config2(seed: int = 0, n: int = 1_000_000, n_sources: int = 10_000, source_f
                      plastic=plastic, name=name)
     net.info["fanout_mean"] = float(fan.mean())
     return net
+
+
+def config2_spec(seed: int = 0, n: int = 1_000_000, n_sources: int = 10_000, source_fanout: int = 1000,
+                 ticks: int = 2000, threshold: float = 8.0, plastic: bool = False, w_exc: float = 0.5,
+                 w_inh: float = 1.0, kick: float = 0.0):
+    """The config-2 network as a device-generator spec plus its input commands, without building the
+    synapse arrays on the host (equal to config2(...) built there). ``kick`` > 0 adds, before the
+    Poisson input, an inject of weight ``threshold`` at tick 0 into each network neuron with
+    probability ``kick``."""
+    fan = lognormal_fanout(seed, n)
+    spec = dict(n=n, n_sources=n_sources, fanout=fan, source_fanout=source_fanout, seed=seed, n_exc=int(round(n * 0.8)),
+                w_exc=w_exc, w_inh=w_inh, delay_max=32, compartments=(BASAL, APICAL), plastic=plastic,
+                threshold=threshold, window=32, sim_thr=0.9, max_refs=1)
+    cmds = poisson_commands(seed, n, n_sources, 0.005, ticks)
+    if kick > 0:
+        hit = np.nonzero(unif24(hash3(seed, 40, np.arange(n, dtype=np.uint64))) < np.float32(kick))[0]
+        cmds = [(0, int(i), 0, float(threshold)) for i in hit] + cmds
+    return spec, cmds

@@ -41,9 +41,7 @@ struct Emu {
     std::vector<Group> groups;
     std::vector<uint32_t> seg_count;
     std::vector<uint64_t> seg_v0;
-    std::vector<uint32_t> seg_n, seg_off, seg_mult, tcnt, toff, rank;
-    std::vector<Entry8> bins8;
-    std::vector<Entry16> bins16;
+    std::vector<uint32_t> seg_n, seg_off, seg_mult;
     std::vector<uint32_t> fire_bits, fire_neuron, fire_mult, fire_word_off, ctl;
     std::vector<uint64_t> totals;
     std::vector<uint32_t> raster;
@@ -73,18 +71,23 @@ void scan_excl(const std::vector<uint32_t>& in, uint32_t n, std::vector<uint32_t
     total = s;
 }
 
-template <bool WIDE, bool CH, bool C2>
-uint32_t scatter_all(Emu& m, const std::vector<std::pair<uint32_t, uint32_t>>& order, uint32_t T) {
-    uint32_t e = 0;
-    for (auto& qi : order) e |= scatter_event<WIDE, CH, C2>(m.S, qi.first, qi.second, T);
-    return e;
-}
-
+// events of the tick in a shuffled order, grouped by target, then folded target by target
 template <bool WIDE, bool ST, bool CH, bool C2>
-uint32_t fold_all(Emu& m, uint32_t T) {
-    uint32_t e = 0;
-    for (uint32_t x = 0; x < m.S.n; ++x) e |= fold_neuron<WIDE, ST, CH, C2>(m.S, x, T);
-    return e;
+uint32_t deliver(Emu& m, const std::vector<std::pair<uint32_t, uint32_t>>& order, uint32_t T) {
+    typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
+    std::vector<uint32_t> xs(order.size());
+    std::vector<E> recs(order.size());
+    uint32_t err = 0;
+    for (size_t k = 0; k < order.size(); ++k) err |= make_event<WIDE, CH, C2>(m.S, order[k].first, order[k].second, T, xs[k], recs[k]);
+    std::vector<uint32_t> cnt(m.S.n + 1, 0);
+    for (uint32_t x : xs) cnt[x + 1]++;
+    for (uint32_t i = 0; i < m.S.n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
+    std::vector<E> by(recs.size());
+    for (size_t k = 0; k < recs.size(); ++k) by[fill[xs[k]]++] = recs[k];
+    for (uint32_t x = 0; x < m.S.n; ++x)
+        err |= fold_events<WIDE, ST, CH, C2>(m.S, by.data() + cnt[x], cnt[x + 1] - cnt[x], x, T);
+    return err;
 }
 
 void tick(Emu& m, uint32_t t0) {
@@ -131,20 +134,12 @@ void tick(Emu& m, uint32_t t0) {
     for (uint32_t q = 0; q < S.seg_count[0]; ++q)
         for (uint32_t i = 0; i < S.seg_n[q]; ++i) order.push_back({q, i});
     std::shuffle(order.begin(), order.end(), m.rng);
-    for (auto& qi : order) count_event(S, qi.first, qi.second);
-    uint32_t tot;
-    scan_excl(m.tcnt, S.n, m.toff, tot);
-    std::shuffle(order.begin(), order.end(), m.rng);
     uint32_t err = 0;
-    if (!m.wide) err |= m.code2 ? scatter_all<false, true, true>(m, order, T) : scatter_all<false, true, false>(m, order, T);
-    else if (m.code_here) err |= m.code2 ? scatter_all<true, true, true>(m, order, T) : scatter_all<true, true, false>(m, order, T);
-    else err |= m.code2 ? scatter_all<true, false, true>(m, order, T) : scatter_all<true, false, false>(m, order, T);
-    // fold
-    if (!m.wide) err |= m.code2 ? fold_all<false, false, true, true>(m, T) : fold_all<false, false, true, false>(m, T);
-    else if (m.stdp && m.code_here) err |= m.code2 ? fold_all<true, true, true, true>(m, T) : fold_all<true, true, true, false>(m, T);
-    else if (m.stdp) err |= m.code2 ? fold_all<true, true, false, true>(m, T) : fold_all<true, true, false, false>(m, T);
-    else if (m.code_here) err |= m.code2 ? fold_all<true, false, true, true>(m, T) : fold_all<true, false, true, false>(m, T);
-    else err |= m.code2 ? fold_all<true, false, false, true>(m, T) : fold_all<true, false, false, false>(m, T);
+    if (!m.wide) err |= m.code2 ? deliver<false, false, true, true>(m, order, T) : deliver<false, false, true, false>(m, order, T);
+    else if (m.stdp && m.code_here) err |= m.code2 ? deliver<true, true, true, true>(m, order, T) : deliver<true, true, true, false>(m, order, T);
+    else if (m.stdp) err |= m.code2 ? deliver<true, true, false, true>(m, order, T) : deliver<true, true, false, false>(m, order, T);
+    else if (m.code_here) err |= m.code2 ? deliver<true, false, true, true>(m, order, T) : deliver<true, false, true, false>(m, order, T);
+    else err |= m.code2 ? deliver<true, false, false, true>(m, order, T) : deliver<true, false, false, false>(m, order, T);
     S.ctl[1] |= err;
     // fire ranks
     const uint32_t nwords = (D + 31) >> 5;
@@ -241,7 +236,7 @@ void* emu_create(uint64_t n, const float* thr, const uint64_t* win, const float*
         S.tgt = h.tgt.data();
         S.w = h.w.data();
         S.meta = h.meta.data();
-        S.delay = h.any_plastic ? h.delay.data() : nullptr;
+        S.delay = h.delay.data();
         S.expected = h.expected.empty() ? nullptr : h.expected.data();
         S.expected_all = h.expected_all ? h.expected_all : 2;
         if (h.code_compact) { m->code2v.assign(h.e + 1, CodeRec2{}); S.code2 = m->code2v.data(); }
@@ -302,14 +297,6 @@ void* emu_create(uint64_t n, const float* thr, const uint64_t* win, const float*
         S.seg_n = m->seg_n.data();
         S.seg_off = m->seg_off.data();
         S.seg_mult = m->seg_mult.data();
-        m->tcnt.assign(h.n, 0);
-        m->toff.assign(h.n, 0);
-        m->rank.assign(S.ev_cap, 0);
-        S.tcnt = m->tcnt.data();
-        S.toff = m->toff.data();
-        S.rank = m->rank.data();
-        if (m->wide) { m->bins16.assign(S.ev_cap, Entry16{}); S.bins = m->bins16.data(); }
-        else { m->bins8.assign(S.ev_cap, Entry8{}); S.bins = m->bins8.data(); }
         m->fire_bits.assign(S.ev_cap / 32 + 1, 0);
         m->fire_neuron.assign(S.ev_cap, 0);
         m->fire_mult.assign(S.ev_cap, 0);
@@ -421,7 +408,7 @@ void emu_synapses(void* p, float* w, uint64_t* pre, uint64_t* cmatch, uint64_t* 
     Emu* m = (Emu*)p;
     const HostNet& h = m->h;
     for (uint64_t j = 0; j < h.e; ++j) {
-        const uint64_t s = h.slot_of[j];
+        const uint32_t s = h.slot_of[j];
         const uint32_t t = h.tpos[s];
         w[j] = t != kNone ? h.stdp[t].weight : h.w[s];
         pre[j] = m->pre_count[s];
