@@ -151,7 +151,7 @@ SNN_HD uint32_t stored_len(const Runs& c, uint32_t nruns) {
 // with matches_learned_pattern (:14-29) and learn_pattern (:31-41). `cand` and `refs` are only
 // dereferenced when needed; they may point to global memory. Returns error flags.
 template <typename CandPtr, typename RefsPtr>
-SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr cand, RefsPtr refs, uint32_t tick,
+SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, RefsPtr refs, uint32_t tick,
                                  const NeuronParams& p, bool& cand_dirty) {
     uint32_t err = 0;
     const bool reset = h.n == 0 || tick - h.start > p.window;

@@ -80,6 +80,9 @@ struct Step {
     uint32_t* upd_count;
     const float* wmax;           // wmax table indexed by StdpRec::aux bits 0-7
     const NeuronParams* np;
+    uint32_t tp_uniform;         // every neuron has the same window / similarity / max_refs
+    uint32_t tp_window, tp_max_refs;
+    float tp_sim;
     NeuronDyn* dyn;
     TemporalHdr* th;
     Runs* cand;
@@ -111,11 +114,8 @@ struct Step {
     // target partitions: partition p holds neurons [p << pshift, (p+1) << pshift)
     uint32_t pshift, nparts, nwg;
     uint32_t* hist;              // [nparts][nwg] per-workgroup event counts, then their bucket bases
-    uint32_t* pgk;               // partition buckets (SoA), event positions [0, D)
-    float* pw;
+    void* prec;                  // partition buckets: Entry8 / Entry16 records, positions [0, D)
     uint16_t* ptl;               // target index within its partition
-    uint32_t* ptpos;             // WIDE only
-    uint32_t* pslot;             // WIDE only
     void* stage;                 // fold scratch for partitions larger than the LDS stage
     uint32_t* fire_bits;
     uint32_t* fire_neuron;
@@ -216,10 +216,11 @@ SNN_HD uint32_t code_expected(const Step& S, uint32_t slot) {
 // segment (first fire of the source this tick) also applies the synapse code pattern `mult` times:
 // the code pattern of a synapse depends only on that synapse's own arrival ticks, and same-tick
 // arrivals are identical. Returns error flags.
+// (v0, g0, mult) are segment q's seg_v0 / seg_off / seg_mult; event i has position g0 + i.
 template <bool WIDE, bool CODE_HERE, bool CODE2, typename E>
-SNN_HD uint32_t make_event(const Step& S, uint32_t q, uint32_t i, uint32_t T, uint32_t& x, E& e) {
-    const uint64_t v0 = S.seg_v0[q];
-    const uint32_t g = S.seg_off[q] + i;
+SNN_HD uint32_t make_event(const Step& S, uint64_t v0, uint32_t g0, uint32_t mult, uint32_t i, uint32_t T,
+                           uint32_t& x, E& e) {
+    const uint32_t g = g0 + i;
     uint32_t err = 0;
     if (v0 & kInjFlag) {
         const uint64_t k = (v0 & ~kInjFlag) + i;
@@ -243,16 +244,37 @@ SNN_HD uint32_t make_event(const Step& S, uint32_t q, uint32_t i, uint32_t T, ui
         e.slot = slot;
     }
     if (CODE_HERE) {
-        const uint32_t mult = S.seg_mult[q];
         if (mult) {
-            const NeuronParams p = S.np[x];
-            if (CODE2) err |= code_observe2(S.code2[slot], T, mult, p.window, p.sim_thr, p.max_refs);
-            else err |= code_observeN(S.codeN[slot], T, mult, code_expected(S, slot), p.window, p.sim_thr, p.max_refs);
+            uint32_t window = S.tp_window, max_refs = S.tp_max_refs;
+            float sim = S.tp_sim;
+            if (!S.tp_uniform) {
+                const NeuronParams p = S.np[x];
+                window = p.window;
+                sim = p.sim_thr;
+                max_refs = p.max_refs;
+            }
+            if (CODE2) err |= code_observe2(S.code2[slot], T, mult, window, sim, max_refs);
+            else err |= code_observeN(S.codeN[slot], T, mult, code_expected(S, slot), window, sim, max_refs);
             if (S.pre_count) S.pre_count[slot] += mult;
         }
     }
     return err;
 }
+
+// A neuron's candidate runs, copied from memory on first use and written back once by the caller.
+struct LazyRuns {
+    Runs* src;
+    Runs r;
+    bool loaded = false;
+    SNN_HD Runs* operator->() {
+        if (!loaded) {
+            r = *src;
+            loaded = true;
+        }
+        return &r;
+    }
+    SNN_HD Runs& operator*() { return *operator->(); }
+};
 
 template <typename E>
 SNN_HD void sort_by_g(E* ev, uint32_t k) {
@@ -310,6 +332,7 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
     TemporalHdr h = S.th[x];
     uint32_t err = 0;
     bool cand_dirty = false;
+    LazyRuns cand{&S.cand[x]};
     decay_to(d, T, S.decay_tab);
     uint32_t fires = 0, first_g = 0;
     for (uint32_t i = 0; i < k; ++i) {
@@ -343,7 +366,7 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
         const float applied_current = w * receptor_gain(rec) * receptor_sign(rec);
         d.cur[comp] += applied_current;
         d.taus = (d.taus & ~(3u << (2 * comp))) | (rec << (2 * comp));
-        err |= temporal_observe(h, &S.cand[x], &S.refs[x], T, p, cand_dirty);
+        err |= temporal_observe(h, cand, &S.refs[x], T, p, cand_dirty);
         d.membrane = d.cur[0] + d.cur[1] + d.cur[2] + d.cur[3];
         if (d.membrane >= p.threshold) {
             d.membrane = 0.0f;
@@ -378,7 +401,7 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
     }
     S.dyn[x] = d;
     S.th[x] = h;
-    (void)cand_dirty;
+    if (cand_dirty) S.cand[x] = cand.r;
     return err;
 }
 

@@ -54,6 +54,7 @@ void check(hipError_t e, const char* what) {
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kScanBlocks = 1024;
+constexpr int kPBlock = 1024;  // partition count / scatter workgroups: 16 waves per CU
 
 // ------------------------------------------------------------------------------------ scans
 
@@ -92,6 +93,25 @@ __device__ uint32_t block_excl_scan(uint32_t v, uint32_t* total) {
     __syncthreads();
     uint32_t base = 0, tot = 0;
     for (int w = 0; w < kBlock / kWave; ++w) {
+        if (w < wid) base += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return base + incl - v;
+}
+
+// exclusive scan across a block of NT threads (one value each)
+template <int NT>
+__device__ uint32_t block_excl_scan_n(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wsum[NT / kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane == kWave - 1) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+    for (int w = 0; w < NT / kWave; ++w) {
         if (w < wid) base += wsum[w];
         tot += wsum[w];
     }
@@ -189,6 +209,7 @@ __global__ void __launch_bounds__(1024) k_plan(Step S) {
             atomicOr(&S.ctl[1], (uint32_t)kErrSegments);
             base = 0;
         }
+        if (S.ctl[1]) base = 0;  // after any error the run is invalid: stop doing work
         S.seg_count[0] = base;
         S.seg_count[1] = 0;
     }
@@ -214,64 +235,106 @@ __global__ void __launch_bounds__(kBlock) k_segments(Step S) {
     }
 }
 
+// Event ranges: workgroup w owns events [D*w/nwg, D*(w+1)/nwg) (a contiguous range of g), caches the
+// descriptors of the segments covering it in LDS and lets its lanes walk consecutive events, so the
+// CSR reads are coalesced and the load is balanced whatever the fan-out distribution.
+constexpr uint32_t kSegChunk = 2048;
+
+struct SegCache {
+    uint32_t off[kSegChunk + 1];
+    uint64_t v0[kSegChunk];
+    uint32_t mult[kSegChunk];
+};
+
+// last segment whose first event is <= g (segments of zero events share their first position)
+__device__ __forceinline__ uint32_t seg_find(const uint32_t* off, uint32_t n, uint32_t g) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= g) lo = mid + 1; else hi = mid;
+    }
+    return lo - 1;
+}
+
+// Calls f(v0, g0, mult, i) for every event of this workgroup's range. Every thread of the block
+// must call it (it synchronises).
+template <bool WITH_MULT, typename F>
+__device__ void for_block_events(const Step& S, SegCache& c, F&& f) {
+    const uint32_t nseg = S.seg_count[0];
+    const uint64_t D = S.seg_count[1];
+    const uint32_t g0 = (uint32_t)(D * blockIdx.x / gridDim.x);
+    const uint32_t g1 = (uint32_t)(D * (blockIdx.x + 1) / gridDim.x);
+    __shared__ uint32_t qstart;
+    if (threadIdx.x == 0 && g0 < g1) qstart = seg_find(S.seg_off, nseg, g0);
+    __syncthreads();
+    if (g0 >= g1) return;
+    uint32_t qa = qstart;
+    while (true) {
+        const uint32_t qb = min(nseg, qa + kSegChunk);
+        const uint32_t nq = qb - qa;
+        for (uint32_t k = threadIdx.x; k < nq; k += blockDim.x) {
+            c.off[k] = S.seg_off[qa + k];
+            c.v0[k] = S.seg_v0[qa + k];
+            if (WITH_MULT) c.mult[k] = S.seg_mult[qa + k];
+        }
+        if (threadIdx.x == 0) c.off[nq] = qb < nseg ? S.seg_off[qb] : (uint32_t)D;
+        __syncthreads();
+        const uint32_t ga = max(g0, c.off[0]);
+        const uint32_t gb = min(g1, c.off[nq]);
+        for (uint32_t g = ga + threadIdx.x; g < gb; g += blockDim.x) {
+            const uint32_t k = seg_find(c.off, nq, g);
+            f(c.v0[k], c.off[k], WITH_MULT ? c.mult[k] : 0u, g - c.off[k]);
+        }
+        __syncthreads();
+        if (gb >= g1 || qb >= nseg) break;
+        qa = qb;
+    }
+}
+
 // Per-workgroup histogram of the tick's events over target partitions (LDS atomics only).
-// The segment -> wave assignment is identical in k_pscatter, which replays the same events.
-__global__ void __launch_bounds__(kBlock) k_pcount(Step S) {
+__global__ void __launch_bounds__(kPBlock) k_pcount(Step S) {
     extern __shared__ uint32_t h[];
+    __shared__ SegCache cache;
     for (uint32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) h[p] = 0;
     __syncthreads();
-    const uint32_t nseg = S.seg_count[0];
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-    for (uint32_t q = wave; q < nseg; q += nwaves) {
-        const uint32_t cnt = S.seg_n[q];
-        const uint64_t v0 = S.seg_v0[q];
-        for (uint32_t i = lane; i < cnt; i += kWave) atomicAdd(&h[event_target(S, v0, i) >> S.pshift], 1u);
-    }
+    const uint32_t shift = S.pshift;
+    for_block_events<false>(S, cache, [&](uint64_t v0, uint32_t, uint32_t, uint32_t i) {
+        atomicAdd(&h[event_target(S, v0, i) >> shift], 1u);
+    });
     __syncthreads();
     for (uint32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) S.hist[(uint64_t)p * S.nwg + blockIdx.x] = h[p];
 }
 
-// Place each event in its partition's bucket at (scanned workgroup base + LDS rank).
+// Place each event in its partition's bucket at (scanned workgroup base + LDS rank). The
+// workgroup -> event assignment is the one k_pcount used.
 template <bool WIDE, bool CODE_HERE, bool CODE2>
-__global__ void __launch_bounds__(kBlock) k_pscatter(Step S) {
+__global__ void __launch_bounds__(kPBlock) k_pscatter(Step S) {
     extern __shared__ uint32_t base[];
+    __shared__ SegCache cache;
     for (uint32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) base[p] = S.hist[(uint64_t)p * S.nwg + blockIdx.x];
     __syncthreads();
     const uint32_t T = S.ctl[0];
-    const uint32_t nseg = S.seg_count[0];
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-    const uint32_t mask = (1u << S.pshift) - 1;
-    uint32_t err = 0;
+    const uint32_t shift = S.pshift;
+    const uint32_t mask = (1u << shift) - 1;
     typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
-    for (uint32_t q = wave; q < nseg; q += nwaves) {
-        const uint32_t cnt = S.seg_n[q];
-        for (uint32_t i = lane; i < cnt; i += kWave) {
-            uint32_t x;
-            E e;
-            err |= make_event<WIDE, CODE_HERE, CODE2>(S, q, i, T, x, e);
-            const uint32_t pos = atomicAdd(&base[x >> S.pshift], 1u);
-            S.pgk[pos] = e.gk;
-            S.pw[pos] = e.w;
-            S.ptl[pos] = (uint16_t)(x & mask);
-            if constexpr (WIDE) {
-                S.ptpos[pos] = e.tpos;
-                S.pslot[pos] = e.slot;
-            }
-        }
-    }
+    uint32_t err = 0;
+    for_block_events<CODE_HERE>(S, cache, [&](uint64_t v0, uint32_t g0, uint32_t mult, uint32_t i) {
+        uint32_t x;
+        E e;
+        err |= make_event<WIDE, CODE_HERE, CODE2>(S, v0, g0, mult, i, T, x, e);
+        const uint32_t pos = atomicAdd(&base[x >> shift], 1u);
+        ((E*)S.prec)[pos] = e;
+        S.ptl[pos] = (uint16_t)(x & mask);
+    });
     if (err) atomicOr(&S.ctl[1], err);
 }
 
-constexpr uint32_t kStageBytes = 32768;
+constexpr uint32_t kStageBytes = 98304;
 
 // One workgroup per target partition: counting sort of the partition's events by target in LDS
 // (global scratch when they do not fit), then one thread per target folds its events.
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
-__global__ void __launch_bounds__(kBlock) k_pfold(Step S) {
+__global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
     typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
     constexpr uint32_t kCap = kStageBytes / sizeof(E);
     __shared__ E lds_stage[kCap];
@@ -293,23 +356,17 @@ __global__ void __launch_bounds__(kBlock) k_pfold(Step S) {
     for (uint32_t t0 = 0; t0 < PS; t0 += blockDim.x) {
         const uint32_t v = cnt[t0 + threadIdx.x];
         uint32_t tot;
-        const uint32_t ex = block_excl_scan(v, &tot);
+        const uint32_t ex = block_excl_scan_n<kPBlock>(v, &tot);
         off[t0 + threadIdx.x] = run + ex;
         fill[t0 + threadIdx.x] = run + ex;
         run += tot;
     }
     __syncthreads();
     E* st = c <= kCap ? lds_stage : (E*)S.stage + start;
+    const E* src = (const E*)S.prec + start;
     for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) {
         const uint32_t pos = atomicAdd(&fill[S.ptl[start + i]], 1u);
-        E e;
-        e.gk = S.pgk[start + i];
-        e.w = S.pw[start + i];
-        if constexpr (WIDE) {
-            e.tpos = S.ptpos[start + i];
-            e.slot = S.pslot[start + i];
-        }
-        st[pos] = e;
+        st[pos] = src[i];
     }
     __syncthreads();
     uint32_t err = 0;
@@ -385,7 +442,7 @@ __global__ void k_advance(Step S, uint32_t t0) {
     S.tick_fired[T - t0] = F;
     S.totals[0] += D;
     S.totals[1] += F;
-    S.totals[2] += F;
+    S.totals[2] += S.tick_fcount[T % (S.lmax + 1)];  // fires actually written to the ring
     if (S.raster && S.totals[3] + F <= S.raster_cap) S.totals[3] += F;
     S.ctl[0] = T + 1;
 }
@@ -689,13 +746,13 @@ void launch_scan(snn_net* net, Src src, uint32_t* out, uint32_t* total, uint64_t
 
 template <bool WIDE, bool CODE_HERE, bool CODE2>
 void launch_scatter(snn_net* net) {
-    hipLaunchKernelGGL((k_pscatter<WIDE, CODE_HERE, CODE2>), dim3(net->S.nwg), dim3(kBlock), net->S.nparts * 4,
+    hipLaunchKernelGGL((k_pscatter<WIDE, CODE_HERE, CODE2>), dim3(net->S.nwg), dim3(kPBlock), net->S.nparts * 4,
                        net->stream, net->S);
 }
 
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
 void launch_fold(snn_net* net) {
-    hipLaunchKernelGGL((k_pfold<WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kBlock),
+    hipLaunchKernelGGL((k_pfold<WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kPBlock),
                        3u * (1u << net->S.pshift) * 4, net->stream, net->S);
 }
 
@@ -773,7 +830,7 @@ void launch_tick(snn_net* net, uint32_t t0, Timer& tm) {
                 std::min<uint64_t>(S.ev_cap, kMaxEventsPerTick), kErrEvents, &S.seg_count[0]);
     tm.end();
     tm.begin(KCount);
-    hipLaunchKernelGGL(k_pcount, dim3(S.nwg), dim3(kBlock), S.nparts * 4, st, S);
+    hipLaunchKernelGGL(k_pcount, dim3(S.nwg), dim3(kPBlock), S.nparts * 4, st, S);
     tm.end();
     tm.begin(KScanTgt);
     launch_scan(net, PlainSrc{S.hist, nullptr, S.nparts * S.nwg}, S.hist, nullptr, 0, 0, nullptr);
@@ -886,10 +943,20 @@ void open_net(snn_net* net, const snn_options& o, uint32_t n, uint64_t e, uint32
 }
 
 // capacities and per-tick work buffers
-void finish_net(snn_net* net, const snn_options& o, uint64_t nve) {
+void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vector<NeuronParams>& params) {
     auto& own = net->owned;
     Step& S = net->S;
     const uint32_t n = S.n;
+    S.tp_uniform = 1;
+    for (const auto& p : params)
+        if (p.window != params[0].window || p.max_refs != params[0].max_refs ||
+            std::memcmp(&p.sim_thr, &params[0].sim_thr, 4) != 0) {
+            S.tp_uniform = 0;
+            break;
+        }
+    S.tp_window = params[0].window;
+    S.tp_max_refs = params[0].max_refs;
+    S.tp_sim = params[0].sim_thr;
     const uint64_t fire_cap = o.fire_log_capacity ? o.fire_log_capacity : std::max<uint64_t>(1u << 16, 8ull * n);
     uint64_t ev_cap = o.max_events_per_tick ? o.max_events_per_tick
                                             : std::max<uint64_t>(1u << 20, std::min<uint64_t>(2 * nve, 1ull << 26));
@@ -909,21 +976,18 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve) {
     S.seg_n = dev_alloc<uint32_t>(own, seg_cap, false);
     S.seg_off = dev_alloc<uint32_t>(own, seg_cap, false);
     S.seg_mult = dev_alloc<uint32_t>(own, seg_cap, false);
-    // target partitions of 256 neurons (more when the partition count would exceed 16384)
-    uint32_t pshift = 8;
+    // target partitions of 1024 neurons (more when the partition count would exceed 16384)
+    uint32_t pshift = 10;
     while (((uint64_t)n + (1ull << pshift) - 1) >> pshift > 16384) ++pshift;
     S.pshift = pshift;
     S.nparts = (uint32_t)(((uint64_t)n + (1ull << pshift) - 1) >> pshift);
     int ncu = 256;
     hipDeviceProp_t pr;
     if (hipGetDeviceProperties(&pr, o.device) == hipSuccess) ncu = pr.multiProcessorCount;
-    S.nwg = (uint32_t)std::max(1, ncu);
+    S.nwg = (uint32_t)std::max(1, 2 * ncu);
     S.hist = dev_alloc<uint32_t>(own, (uint64_t)S.nparts * S.nwg);
-    S.pgk = dev_alloc<uint32_t>(own, ev_cap, false);
-    S.pw = dev_alloc<float>(own, ev_cap, false);
+    S.prec = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
     S.ptl = dev_alloc<uint16_t>(own, ev_cap, false);
-    S.ptpos = net->wide ? dev_alloc<uint32_t>(own, ev_cap, false) : nullptr;
-    S.pslot = net->wide ? dev_alloc<uint32_t>(own, ev_cap, false) : nullptr;
     S.stage = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
     S.fire_bits = dev_alloc<uint32_t>(own, ev_cap / 32);
     S.fire_neuron = dev_alloc<uint32_t>(own, ev_cap, false);
@@ -1015,7 +1079,7 @@ int snn_net_create(const snn_net_desc* desc, const snn_options* opts, snn_net** 
         }
         S.np = dev_upload(own, h.params);
         net->slot_of = dev_upload(own, h.slot_of);
-        finish_net(net.get(), o, h.nve);
+        finish_net(net.get(), o, h.nve, h.params);
         *out = net.release();
         return SNN_OK;
     });
@@ -1129,7 +1193,7 @@ int snn_net_create_ei(const snn_ei_desc* g, const snn_options* opts, snn_net** o
         }
         S.np = dev_upload(own, params);
         check(hipStreamSynchronize(st), "generate");
-        finish_net(net.get(), o, e);
+        finish_net(net.get(), o, e, params);
         *out = net.release();
         return SNN_OK;
     });

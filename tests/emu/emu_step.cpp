@@ -78,7 +78,10 @@ uint32_t deliver(Emu& m, const std::vector<std::pair<uint32_t, uint32_t>>& order
     std::vector<uint32_t> xs(order.size());
     std::vector<E> recs(order.size());
     uint32_t err = 0;
-    for (size_t k = 0; k < order.size(); ++k) err |= make_event<WIDE, CH, C2>(m.S, order[k].first, order[k].second, T, xs[k], recs[k]);
+    for (size_t k = 0; k < order.size(); ++k) {
+        const uint32_t q = order[k].first;
+        err |= make_event<WIDE, CH, C2>(m.S, m.S.seg_v0[q], m.S.seg_off[q], m.S.seg_mult[q], order[k].second, T, xs[k], recs[k]);
+    }
     std::vector<uint32_t> cnt(m.S.n + 1, 0);
     for (uint32_t x : xs) cnt[x + 1]++;
     for (uint32_t i = 0; i < m.S.n; ++i) cnt[i + 1] += cnt[i];
@@ -252,6 +255,13 @@ void* emu_create(uint64_t n, const float* thr, const uint64_t* win, const float*
             S.wmax = h.wmax.data();
         }
         S.np = h.params.data();
+        S.tp_uniform = 1;
+        for (const auto& q : h.params)
+            if (q.window != h.params[0].window || q.max_refs != h.params[0].max_refs || q.sim_thr != h.params[0].sim_thr)
+                S.tp_uniform = 0;
+        S.tp_window = h.params[0].window;
+        S.tp_max_refs = h.params[0].max_refs;
+        S.tp_sim = h.params[0].sim_thr;
         m->dyn.assign(h.n, NeuronDyn{});
         m->th.assign(h.n, TemporalHdr{});
         m->cand.assign(h.n, Runs{});

@@ -77,8 +77,9 @@ def test_split_runs_equal_one_run():
 @pytest.mark.parametrize("plastic", [False, True])
 def test_device_generator_equals_host_build(plastic):
     # snn_net_create_ei must build the same network as snn_net_create on workloads.config2's arrays
-    net = W.config2(3, n=3000, n_sources=60, source_fanout=200, ticks=120, threshold=1.0, plastic=plastic)
-    net.commands = W.poisson_commands(3, 3000, 60, 0.05, 120)
+    net = W.ei_network(3000, W.lognormal_fanout(3, 3000), 3, n_sources=60, source_fanout=200, p_source=0.05,
+                       ticks=120, threshold=2.5, delay_max=32, compartments=(W.BASAL, W.APICAL), plastic=plastic,
+                       w_inh=5.0)
     a = run_device(net)
     b = sa.Network(spec=net.info["spec"])
     b.run_commands(net.commands)
