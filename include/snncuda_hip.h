@@ -118,7 +118,9 @@ int snn_net_counts(snn_net* net, uint64_t* delivered, uint64_t* fired, uint64_t*
 int snn_net_read_tick_counts(snn_net* net, uint64_t first_tick, uint64_t n, uint64_t* delivered, uint64_t* fired);
 
 /* Fires in processing order (tick-major; within a tick in the reference's order). *n_out receives
- * the number recorded; neurons (may be NULL) receives min(n_out, cap) neuron indices. */
+ * the number recorded; neurons (may be NULL) receives min(n_out, cap) neuron indices. Recording stops
+ * (the run does not) once raster_capacity is reached; the call then returns SNN_ECAPACITY after
+ * filling what was recorded. */
 int snn_net_read_raster(snn_net* net, uint64_t* n_out, uint32_t* neurons, uint64_t cap);
 
 /* Final neuron state (CudaPropagationResult fields, reference include/snncuda/backends/CudaBackend.h:40-53,

@@ -310,9 +310,16 @@ SNN_HD uint32_t code_observe2(CodeRec2& r, uint32_t tick, uint32_t times, uint32
         if (nobs < 2) continue;
         float best = 0.0f;
         if (nlearned) {
-            const uint16_t a[2] = {(uint16_t)f0, (uint16_t)f1};
-            const uint16_t b[2] = {(uint16_t)o0, (uint16_t)o1};
-            const float c = cosine_u16(a, b, 2);
+            // cosine_u16 of (f0, f1) and (o0, o1), written out (index order, FP64)
+            const double a0 = (double)f0, a1 = (double)f1, b0 = (double)o0, b1 = (double)o1;
+            double dot = 0.0, ln = 0.0, rn = 0.0;
+            dot += a0 * b0;
+            ln += a0 * a0;
+            rn += b0 * b0;
+            dot += a1 * b1;
+            ln += a1 * a1;
+            rn += b1 * b1;
+            const float c = (ln == 0.0 || rn == 0.0) ? 0.0f : (float)(dot / (sqrt(ln) * sqrt(rn)));
             best = (best < c) ? c : best;
         }
         if (best >= sim_thr) {
@@ -421,6 +428,7 @@ SNN_HD bool stdp_apply(StdpRec& r, uint32_t pre, uint32_t post, const float* ltp
 SNN_HD void decay_to(NeuronDyn& d, uint32_t tick, const float* decay_tab) {
     if (tick > d.last_tick) {
         const uint32_t dt = tick - d.last_tick;
+#pragma unroll
         for (uint32_t k = 0; k < 4; ++k) {
             if (d.cur[k] != 0.0f) {
                 const uint32_t code = (d.taus >> (2 * k)) & 3u;

@@ -122,7 +122,7 @@ struct Step {
     uint32_t* fire_mult;
     uint32_t* fire_word_off;
     // device state
-    uint32_t* ctl;               // [0] tick, [1] error flags, [2] fires this tick, [3] pad
+    uint32_t* ctl;               // [0] tick, [1] error flags, [2] fires this tick, [3] bit 0: raster full
     uint64_t* totals;            // [0] delivered, [1] fired, [2] fire-log head, [3] raster fill
     uint32_t* tick_delivered;    // per tick of the run segment
     uint32_t* tick_fired;
@@ -261,27 +261,14 @@ SNN_HD uint32_t make_event(const Step& S, uint64_t v0, uint32_t g0, uint32_t mul
     return err;
 }
 
-// A neuron's candidate runs, copied from memory on first use and written back once by the caller.
-struct LazyRuns {
-    Runs* src;
-    Runs r;
-    bool loaded = false;
-    SNN_HD Runs* operator->() {
-        if (!loaded) {
-            r = *src;
-            loaded = true;
-        }
-        return &r;
-    }
-    SNN_HD Runs& operator*() { return *operator->(); }
-};
-
 template <typename E>
 SNN_HD void sort_by_g(E* ev, uint32_t k) {
-    // Shell sort (Ciura gaps) in place; k is small for almost every target.
-    const uint32_t gaps[8] = {701, 301, 132, 57, 23, 10, 4, 1};
+    // Shell sort (Ciura gaps) in place; k is small for almost every target. The gap loop is
+    // unrolled so the gap table stays in registers.
+#pragma unroll
     for (uint32_t gi = 0; gi < 8; ++gi) {
-        const uint32_t gap = gaps[gi];
+        const uint32_t gap = gi == 0 ? 701u : gi == 1 ? 301u : gi == 2 ? 132u : gi == 3 ? 57u
+                           : gi == 4 ? 23u : gi == 5 ? 10u : gi == 6 ? 4u : 1u;
         if (gap >= k) continue;
         for (uint32_t i = gap; i < k; ++i) {
             const E t = ev[i];
@@ -332,7 +319,7 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
     TemporalHdr h = S.th[x];
     uint32_t err = 0;
     bool cand_dirty = false;
-    LazyRuns cand{&S.cand[x]};
+    Runs* cand = &S.cand[x];
     decay_to(d, T, S.decay_tab);
     uint32_t fires = 0, first_g = 0;
     for (uint32_t i = 0; i < k; ++i) {
@@ -364,7 +351,11 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
         }
         // process_input (:175-212)
         const float applied_current = w * receptor_gain(rec) * receptor_sign(rec);
-        d.cur[comp] += applied_current;
+        // cur[comp] += applied, written as selects so the compartments stay in registers
+        d.cur[0] = comp == 0 ? d.cur[0] + applied_current : d.cur[0];
+        d.cur[1] = comp == 1 ? d.cur[1] + applied_current : d.cur[1];
+        d.cur[2] = comp == 2 ? d.cur[2] + applied_current : d.cur[2];
+        d.cur[3] = comp == 3 ? d.cur[3] + applied_current : d.cur[3];
         d.taus = (d.taus & ~(3u << (2 * comp))) | (rec << (2 * comp));
         err |= temporal_observe(h, cand, &S.refs[x], T, p, cand_dirty);
         d.membrane = d.cur[0] + d.cur[1] + d.cur[2] + d.cur[3];
@@ -401,7 +392,6 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
     }
     S.dyn[x] = d;
     S.th[x] = h;
-    if (cand_dirty) S.cand[x] = cand.r;
     return err;
 }
 

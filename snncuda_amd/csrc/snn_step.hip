@@ -388,7 +388,8 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
     const uint64_t oldest = (S.lmax == 1) ? head : (T + 1 >= S.lmax ? S.tick_fstart[(T + 1 - S.lmax) % R] : 0);
     const bool fits = head + F - oldest <= S.fire_cap;
     const uint64_t rfill = S.totals[3];
-    const bool rfits = S.raster && rfill + F <= S.raster_cap;
+    // the raster is a recording aid: when it is full, recording stops and the run goes on
+    const bool rfits = S.raster && !(S.ctl[3] & 1u) && rfill + F <= S.raster_cap;
     for (uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x; wi < nwords; wi += gridDim.x * blockDim.x) {
         uint32_t bits = S.fire_bits[wi];
         if (!bits) continue;
@@ -409,7 +410,7 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (!fits) atomicOr(&S.ctl[1], (uint32_t)kErrFireLog);
-        if (S.raster && !rfits) atomicOr(&S.ctl[1], (uint32_t)kErrRaster);
+        if (S.raster && !rfits) atomicOr(&S.ctl[3], 1u);
         S.tick_fstart[T % R] = head;
         S.tick_fcount[T % R] = fits ? F : 0;
     }
@@ -443,7 +444,7 @@ __global__ void k_advance(Step S, uint32_t t0) {
     S.totals[0] += D;
     S.totals[1] += F;
     S.totals[2] += S.tick_fcount[T % (S.lmax + 1)];  // fires actually written to the ring
-    if (S.raster && S.totals[3] + F <= S.raster_cap) S.totals[3] += F;
+    if (S.raster && !(S.ctl[3] & 1u)) S.totals[3] += F;
     S.ctl[0] = T + 1;
 }
 
@@ -1326,11 +1327,14 @@ int snn_net_read_raster(snn_net* net, uint64_t* n_out, uint32_t* neurons, uint64
         check(hipSetDevice(net->device), "hipSetDevice");
         uint64_t tot[4];
         check(hipMemcpy(tot, net->S.totals, sizeof(tot), hipMemcpyDeviceToHost), "read totals");
+        uint32_t ctl[4];
+        check(hipMemcpy(ctl, net->S.ctl, sizeof(ctl), hipMemcpyDeviceToHost), "read ctl");
         *n_out = tot[3];
         if (neurons && tot[3]) {
             const uint64_t m = std::min(cap, tot[3]);
             check(hipMemcpy(neurons, net->S.raster, m * 4, hipMemcpyDeviceToHost), "read raster");
         }
+        if (ctl[3] & 1u) return fail(SNN_ECAPACITY, "raster capacity reached: only the first fires were recorded");
         return SNN_OK;
     });
 }

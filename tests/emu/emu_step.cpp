@@ -157,7 +157,7 @@ void tick(Emu& m, uint32_t t0) {
     const uint64_t oldest = (S.lmax == 1) ? head : (T + 1 >= S.lmax ? S.tick_fstart[(T + 1 - S.lmax) % R] : 0);
     const bool fits = head + F - oldest <= S.fire_cap;
     const uint64_t rfill = S.totals[3];
-    const bool rfits = rfill + F <= S.raster_cap;
+    const bool rfits = !(S.ctl[3] & 1u) && rfill + F <= S.raster_cap;
     for (uint32_t wi = 0; wi < nwords; ++wi) {
         uint32_t bits = S.fire_bits[wi];
         if (!bits) continue;
@@ -177,7 +177,7 @@ void tick(Emu& m, uint32_t t0) {
         }
     }
     if (!fits) S.ctl[1] |= kErrFireLog;
-    if (!rfits) S.ctl[1] |= kErrRaster;
+    if (!rfits) S.ctl[3] |= 1u;
     S.tick_fstart[T % R] = head;
     S.tick_fcount[T % R] = fits ? F : 0;
     // post-synaptic scans
@@ -197,7 +197,7 @@ void tick(Emu& m, uint32_t t0) {
     S.totals[0] += D;
     S.totals[1] += F;
     S.totals[2] += F;
-    if (S.totals[3] + F <= S.raster_cap) S.totals[3] += F;
+    if (!(S.ctl[3] & 1u)) S.totals[3] += F;
     S.ctl[0] = T + 1;
 }
 
