@@ -42,9 +42,6 @@ SCATTER_BYTES_PER_EVENT = 10.0 + 32.0
 # STDP on adds 12 B per event (pre-side record) and 20 B per (fire, incoming plastic synapse)
 STDP_EVENT_BYTES = 12.0
 STDP_POST_BYTES = 20.0
-# rocprofv3 FETCH_SIZE + WRITE_SIZE of a plain copy kernel of known size, divided by the bytes it
-# moves (profiles/copy_calibration.txt); multiply raw counter sums by this
-PMC_SCALE = 1.0
 
 
 def parse():
@@ -130,7 +127,9 @@ def main():
     if tr_file.exists():
         tr = json.loads(tr_file.read_text())
         if tr.get("stdp", False) == args.stdp:
-            traffic = tr["bytes_per_launch"] * PMC_SCALE
+            # counters scaled by the copy-kernel calibration (profiles/copy_calibration.txt), per
+            # delivered event, times this run's events per launch
+            traffic = tr["bytes_per_event"] * ev_per_launch
     ms_per_step = elapsed / args.steps * 1e3
     step_gbs = STEP_BYTES_PER_EVENT * events / elapsed / 1e9
     line = {

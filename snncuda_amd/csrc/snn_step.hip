@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -330,27 +331,32 @@ __global__ void __launch_bounds__(kPBlock) k_pscatter(Step S) {
 }
 
 constexpr uint32_t kStageBytes = 98304;
+constexpr uint32_t kRound = kPBlock;  // targets folded per round (one per thread)
 
-// One workgroup per target partition: counting sort of the partition's events by target in LDS
-// (global scratch when they do not fit), then one thread per target folds its events.
+// One workgroup per target partition. The partition's targets are folded in rounds of 1024 (one
+// per thread): a counting sort by target over the whole partition gives every target's place, and
+// each round stages only its own targets' records in LDS (global scratch if they do not fit), so
+// the bucket is read once for its records and once per round for the 2-byte target indices.
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
 __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
     typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
     constexpr uint32_t kCap = kStageBytes / sizeof(E);
     __shared__ E lds_stage[kCap];
+    __shared__ uint32_t fill[kRound];
     extern __shared__ uint32_t sh[];
     const uint32_t PS = 1u << S.pshift;
     uint32_t* cnt = sh;
-    uint32_t* off = sh + PS;
-    uint32_t* fill = sh + 2 * PS;
+    uint32_t* off = sh + PS;  // PS + 1 entries
     const uint32_t p = blockIdx.x;
     const uint32_t T = S.ctl[0];
     const uint32_t start = S.hist[(uint64_t)p * S.nwg];
     const uint32_t end = p + 1 < S.nparts ? S.hist[(uint64_t)(p + 1) * S.nwg] : S.seg_count[1];
     const uint32_t c = end - start;
+    const uint16_t* tl = S.ptl + start;
+    const E* src = (const E*)S.prec + start;
     for (uint32_t t = threadIdx.x; t < PS; t += blockDim.x) cnt[t] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) atomicAdd(&cnt[S.ptl[start + i]], 1u);
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) atomicAdd(&cnt[tl[i]], 1u);
     __syncthreads();
     uint32_t run = 0;
     for (uint32_t t0 = 0; t0 < PS; t0 += blockDim.x) {
@@ -358,21 +364,28 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
         uint32_t tot;
         const uint32_t ex = block_excl_scan_n<kPBlock>(v, &tot);
         off[t0 + threadIdx.x] = run + ex;
-        fill[t0 + threadIdx.x] = run + ex;
         run += tot;
     }
-    __syncthreads();
-    E* st = c <= kCap ? lds_stage : (E*)S.stage + start;
-    const E* src = (const E*)S.prec + start;
-    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) {
-        const uint32_t pos = atomicAdd(&fill[S.ptl[start + i]], 1u);
-        st[pos] = src[i];
-    }
+    if (threadIdx.x == 0) off[PS] = run;
     __syncthreads();
     uint32_t err = 0;
-    for (uint32_t t = threadIdx.x; t < PS; t += blockDim.x) {
+    const uint32_t rounds = PS / kRound;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t lo = off[r * kRound], hi = off[(r + 1) * kRound];
+        const uint32_t t = r * kRound + threadIdx.x;
+        fill[threadIdx.x] = off[t] - lo;
+        __syncthreads();
+        E* st = hi - lo <= kCap ? lds_stage : (E*)S.stage + start + lo;
+        for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) {
+            const uint32_t ti = tl[i];
+            if (ti / kRound != r) continue;
+            const uint32_t pos = atomicAdd(&fill[ti % kRound], 1u);
+            st[pos] = src[i];
+        }
+        __syncthreads();
         const uint32_t x = (p << S.pshift) + t;
-        if (x < S.n && cnt[t]) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2>(S, st + off[t], cnt[t], x, T);
+        if (x < S.n && cnt[t]) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2>(S, st + (off[t] - lo), cnt[t], x, T);
+        __syncthreads();
     }
     if (err) atomicOr(&S.ctl[1], err);
 }
@@ -754,7 +767,7 @@ void launch_scatter(snn_net* net) {
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
 void launch_fold(snn_net* net) {
     hipLaunchKernelGGL((k_pfold<WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kPBlock),
-                       3u * (1u << net->S.pshift) * 4, net->stream, net->S);
+                       (2u * (1u << net->S.pshift) + 1) * 4, net->stream, net->S);
 }
 
 void dispatch_scatter(snn_net* net) {
@@ -977,15 +990,20 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     S.seg_n = dev_alloc<uint32_t>(own, seg_cap, false);
     S.seg_off = dev_alloc<uint32_t>(own, seg_cap, false);
     S.seg_mult = dev_alloc<uint32_t>(own, seg_cap, false);
-    // target partitions of 1024 neurons (more when the partition count would exceed 16384)
-    uint32_t pshift = 10;
-    while (((uint64_t)n + (1ull << pshift) - 1) >> pshift > 16384) ++pshift;
+    // Target partitions of 2048 neurons: few enough buckets that each scatter workgroup's runs per
+    // bucket stay long (writes coalesce in L2); folded in rounds of 1024 neurons. Networks of at
+    // most 1024 neurons use a single 1024-neuron partition.
+    uint32_t pshift = n <= 1024 ? 10 : 11;
+    if (const char* e = std::getenv("SNNCUDA_PART_SHIFT")) pshift = std::max(10, std::min(12, std::atoi(e)));
+    if (((uint64_t)n + (1ull << pshift) - 1) >> pshift > 65535)
+        throw std::invalid_argument("too many neurons for 4096-neuron partitions on one device");
     S.pshift = pshift;
     S.nparts = (uint32_t)(((uint64_t)n + (1ull << pshift) - 1) >> pshift);
     int ncu = 256;
     hipDeviceProp_t pr;
     if (hipGetDeviceProperties(&pr, o.device) == hipSuccess) ncu = pr.multiProcessorCount;
-    S.nwg = (uint32_t)std::max(1, 2 * ncu);
+    S.nwg = (uint32_t)std::max(1, ncu);  // one scatter workgroup per CU (measured best with 1024-neuron partitions)
+    if (const char* e = std::getenv("SNNCUDA_SCATTER_WGS")) S.nwg = (uint32_t)std::max(1, std::atoi(e));
     S.hist = dev_alloc<uint32_t>(own, (uint64_t)S.nparts * S.nwg);
     S.prec = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
     S.ptl = dev_alloc<uint16_t>(own, ev_cap, false);

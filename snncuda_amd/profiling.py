@@ -62,12 +62,14 @@ def cmd_calibrate(a):
     name = max(f, key=lambda k: statistics.mean(f[k]))
     fk = statistics.median(f[name][1:] or f[name])
     wk = statistics.median(w[name][1:] or w[name])
-    moved_kib = 2 * a.mib * 1024
-    scale = moved_kib / (fk + wk)
-    text = (f"copy calibration: kernel {name}, {a.mib} MiB in + {a.mib} MiB out per dispatch\n"
-            f"FETCH_SIZE median {fk:.1f} KiB, WRITE_SIZE median {wk:.1f} KiB, sum {fk + wk:.1f} KiB\n"
-            f"bytes moved {moved_kib} KiB -> scale factor (moved / counted) = {scale:.4f}\n")
+    moved_kib = a.mib * 1024
+    fs, ws = moved_kib / fk, moved_kib / wk
+    text = (f"copy calibration: kernel {name}, {a.mib} MiB read + {a.mib} MiB written per dispatch\n"
+            f"FETCH_SIZE median {fk:.1f} KiB -> fetch scale (bytes read / counted) = {fs:.4f}\n"
+            f"WRITE_SIZE median {wk:.1f} KiB -> write scale (bytes written / counted) = {ws:.4f}\n")
     open(a.out, "w").write(text)
+    if a.json:
+        open(a.json, "w").write(json.dumps({"fetch_scale": fs, "write_scale": ws, "mib": a.mib}, indent=1) + "\n")
     print(text)
 
 
@@ -95,6 +97,11 @@ def cmd_summary(a):
             lines.append(f"{k[:58]:58s} {fv:12.1f} {wv:12.1f}")
             if a.kernel and k.startswith(a.kernel):
                 result = {"kernel": k, "fetch_kib": fv, "write_kib": wv, "bytes_per_launch": (fv + wv) * 1024}
+    if result and a.calibration:
+        cal = json.load(open(a.calibration))
+        result["fetch_scale"], result["write_scale"] = cal["fetch_scale"], cal["write_scale"]
+        result["bytes_per_launch"] = (result["fetch_kib"] * cal["fetch_scale"] + result["write_kib"] * cal["write_scale"]) * 1024
+        lines += ["", f"scaled by the copy calibration: fetch x{cal['fetch_scale']:.3f}, write x{cal['write_scale']:.3f}"]
     if a.kernel and a.events_per_launch and result:
         result["events_per_launch"] = a.events_per_launch
         result["bytes_per_event"] = result["bytes_per_launch"] / a.events_per_launch
@@ -119,6 +126,7 @@ def main(argv=None):
     k.add_argument("--write", required=True)
     k.add_argument("--mib", type=int, default=1024)
     k.add_argument("--out", required=True)
+    k.add_argument("--json")
     s = sub.add_parser("summary")
     s.add_argument("--trace", required=True)
     s.add_argument("--fetch")
@@ -130,6 +138,7 @@ def main(argv=None):
     s.add_argument("--cmd", required=True)
     s.add_argument("--out", required=True)
     s.add_argument("--json")
+    s.add_argument("--calibration")
     a = ap.parse_args(argv)
     {"copy": cmd_copy, "calibrate": cmd_calibrate, "summary": cmd_summary}[a.cmd_name](a)
 
