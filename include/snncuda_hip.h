@@ -102,8 +102,6 @@ typedef struct snn_ei_desc {
 } snn_ei_desc;
 int snn_net_create_ei(const snn_ei_desc* desc, const snn_options* opts, snn_net** out);
 
-
-
 /* NetworkPropagator::inject (src/runtime/NetworkPropagator.cpp:31-37), n events in call order. */
 int snn_net_inject(snn_net* net, const uint32_t* neuron, const uint64_t* tick, const float* weight, uint64_t n);
 
@@ -133,6 +131,12 @@ int snn_net_read_neurons(snn_net* net, float* membrane, float* currents, float* 
  * last_pre / last_post are -1 when unset. pre_count needs track_pre_count. */
 int snn_net_read_synapses(snn_net* net, float* weight, uint64_t* pre_count, uint64_t* update_count,
                           int64_t* last_pre, int64_t* last_post, uint64_t* code_match, uint64_t* code_learned);
+
+/* Changed STDP updates per synapse in connectome order, split by the sign of the applied delta
+ * (delta > 0: LTP; otherwise LTD), as the reference's stdp_ltp / stdp_ltd diagnostic counters split
+ * them (reference src/backends/cuda/CudaBackend.cu:133-139). ltp + ltd equals update_count of
+ * snn_net_read_synapses. Either pointer may be NULL. */
+int snn_net_read_plasticity(snn_net* net, uint64_t* ltp_updates, uint64_t* ltd_updates);
 
 /* Mean duration (ms) of each step kernel over the last run, measured with HIP events on the
  * handle's stream when profiling is enabled (snn_net_set_profiling). names receives a

@@ -1,0 +1,325 @@
+// C++ layer over the C ABI in snncuda_hip.h: an RAII handle that rethrows status codes as
+// exceptions, a packer from any Connectome-shaped type to snn_net_desc, and the body of
+// CudaBackend::run_resident_propagation with the reference CPU semantics.
+//
+// Header-only and free of the reference's headers: the packer and the propagation function are
+// templates over the reference types (reference include/snncuda/declarative/Connectome.h:13-37,
+// include/snncuda/backends/CudaBackend.h:14-53), so the same code compiles inside the reference
+// tree (snncuda_amd/cpp/CudaBackendHip.cpp) and anywhere else with structs of the same shape.
+#ifndef SNNCUDA_HIP_HPP
+#define SNNCUDA_HIP_HPP
+
+#include "snncuda_hip.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <utility>
+#include <vector>
+
+namespace snncuda_hip {
+
+// Status codes from the C ABI become exceptions here, as the reference's check_cuda raises
+// std::runtime_error (reference src/backends/cuda/CudaBackend.cu:47-51).
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& what) : std::runtime_error(what), code_(code) {}
+    [[nodiscard]] int code() const noexcept { return code_; }
+
+private:
+    int code_;
+};
+
+inline void check(int rc, const char* call) {
+    if (rc != SNN_OK) {
+        throw Error(rc, std::string(call) + ": " + snn_last_error());
+    }
+}
+
+// One device-resident network (move-only). The state persists across run_until calls, like
+// runtime::NetworkPropagator (reference include/snncuda/runtime/NetworkPropagator.h:14-47).
+class Network {
+public:
+    Network() = default;
+    explicit Network(const snn_net_desc& desc, const snn_options* opts = nullptr) {
+        check(snn_net_create(&desc, opts, &handle_), "snn_net_create");
+    }
+    ~Network() { reset(); }
+    Network(const Network&) = delete;
+    Network& operator=(const Network&) = delete;
+    Network(Network&& o) noexcept : handle_(std::exchange(o.handle_, nullptr)) {}
+    Network& operator=(Network&& o) noexcept {
+        if (this != &o) {
+            reset();
+            handle_ = std::exchange(o.handle_, nullptr);
+        }
+        return *this;
+    }
+
+    void inject(std::uint32_t neuron, std::uint64_t tick, float weight) {
+        check(snn_net_inject(handle_, &neuron, &tick, &weight, 1), "snn_net_inject");
+    }
+    void inject(const std::vector<std::uint32_t>& neuron, const std::vector<std::uint64_t>& tick,
+                const std::vector<float>& weight) {
+        if (neuron.size() != tick.size() || neuron.size() != weight.size()) {
+            throw Error(SNN_EINVAL, "inject: neuron, tick and weight differ in length");
+        }
+        check(snn_net_inject(handle_, neuron.data(), tick.data(), weight.data(), neuron.size()),
+              "snn_net_inject");
+    }
+    void run_until(std::uint64_t inclusive_tick) {
+        check(snn_net_run_until(handle_, inclusive_tick), "snn_net_run_until");
+    }
+    [[nodiscard]] std::uint64_t delivered() const { return counts().delivered; }
+    [[nodiscard]] std::uint64_t fired() const { return counts().fired; }
+
+    struct Counts {
+        std::uint64_t delivered, fired, next_tick;
+    };
+    [[nodiscard]] Counts counts() const {
+        Counts c{};
+        check(snn_net_counts(handle_, &c.delivered, &c.fired, &c.next_tick), "snn_net_counts");
+        return c;
+    }
+    [[nodiscard]] snn_net* get() const noexcept { return handle_; }
+
+private:
+    void reset() noexcept {
+        if (handle_) {
+            snn_net_destroy(handle_);
+            handle_ = nullptr;
+        }
+    }
+    snn_net* handle_{nullptr};
+};
+
+// A Connectome flattened into the SoA arrays snn_net_desc points into, in connectome order.
+// Neuron ids are mapped to dense indices in neuron order; a synapse whose source or target is not
+// a neuron of the connectome is rejected (the device layout has no slot for it).
+struct PackedConnectome {
+    std::vector<float> threshold, sim_thr;
+    std::vector<std::uint64_t> window, max_refs;
+    std::vector<std::uint32_t> source, target, delay, codes;
+    std::vector<float> weight, max_weight;
+    std::vector<std::uint8_t> compartment, receptor, plastic;
+    std::vector<std::uint64_t> code_ptr;
+    std::unordered_map<std::uint64_t, std::uint32_t> index;  // NeuronId -> dense index
+
+    [[nodiscard]] snn_net_desc desc() const {
+        snn_net_desc d{};
+        d.n_neurons = threshold.size();
+        d.threshold = threshold.data();
+        d.window = window.data();
+        d.sim_thr = sim_thr.data();
+        d.max_refs = max_refs.data();
+        d.n_synapses = source.size();
+        d.source = source.data();
+        d.target = target.data();
+        d.weight = weight.data();
+        d.max_weight = max_weight.data();
+        d.delay = delay.data();
+        d.compartment = compartment.data();
+        d.receptor = receptor.data();
+        d.plastic = plastic.data();
+        d.code_ptr = code_ptr.data();
+        d.codes = codes.data();
+        return d;
+    }
+};
+
+// Connectome: .neurons[i].{id, params.{threshold, pattern_window_ticks, similarity_threshold,
+// max_reference_patterns}}, .synapses[j].{source, target, weight, max_weight, delay_ticks,
+// spike_code_offsets, compartment, receptor, plasticity_enabled}.
+template <class Connectome>
+PackedConnectome pack(const Connectome& c) {
+    PackedConnectome p;
+    const std::size_t n = c.neurons.size();
+    const std::size_t e = c.synapses.size();
+    if (n > 0xFFFFFFFEull) {
+        throw Error(SNN_EINVAL, "pack: more than 2^32-1 neurons");
+    }
+    p.threshold.resize(n);
+    p.window.resize(n);
+    p.sim_thr.resize(n);
+    p.max_refs.resize(n);
+    p.index.reserve(n);
+    for (std::size_t i = 0; i < n; ++i) {
+        const auto& nr = c.neurons[i];
+        p.index.emplace(static_cast<std::uint64_t>(nr.id), static_cast<std::uint32_t>(i));
+        p.threshold[i] = nr.params.threshold;
+        p.window[i] = nr.params.pattern_window_ticks;
+        p.sim_thr[i] = nr.params.similarity_threshold;
+        p.max_refs[i] = static_cast<std::uint64_t>(nr.params.max_reference_patterns);
+    }
+    p.source.resize(e);
+    p.target.resize(e);
+    p.weight.resize(e);
+    p.max_weight.resize(e);
+    p.delay.resize(e);
+    p.compartment.resize(e);
+    p.receptor.resize(e);
+    p.plastic.resize(e);
+    p.code_ptr.resize(e + 1);
+    p.code_ptr[0] = 0;
+    for (std::size_t j = 0; j < e; ++j) {
+        const auto& s = c.synapses[j];
+        const auto src = p.index.find(static_cast<std::uint64_t>(s.source));
+        const auto tgt = p.index.find(static_cast<std::uint64_t>(s.target));
+        if (src == p.index.end() || tgt == p.index.end()) {
+            throw Error(SNN_EINVAL, "pack: synapse " + std::to_string(j) + " names a neuron id not in the connectome");
+        }
+        p.source[j] = src->second;
+        p.target[j] = tgt->second;
+        p.weight[j] = s.weight;
+        p.max_weight[j] = s.max_weight;
+        p.delay[j] = s.delay_ticks;
+        p.compartment[j] = static_cast<std::uint8_t>(s.compartment);
+        p.receptor[j] = static_cast<std::uint8_t>(s.receptor);
+        p.plastic[j] = s.plasticity_enabled ? 1 : 0;
+        for (const auto off : s.spike_code_offsets) {
+            p.codes.push_back(static_cast<std::uint32_t>(off));
+        }
+        p.code_ptr[j + 1] = p.codes.size();
+    }
+    return p;
+}
+
+// CudaBackend::run_resident_propagation (reference include/snncuda/backends/CudaBackend.h:134-137)
+// with the reference CPU semantics, which the reference's own CUDA tests compare against
+// (reference tests/cuda_synapse_dendrite_tests.cpp:122-130): every distinct, known id of
+// initially_active receives inject(id, 0, 1.0) in list order, then ticks 0..max_steps run.
+// Unknown ids are skipped and duplicates count once, as in the reference (CudaBackend.cu:3480-3491);
+// an empty connectome or an empty active set gives executed=true with empty results
+// (CudaBackend.cu:3378-3380, 3489-3491).
+//
+// Result is CudaPropagationResult-shaped (CudaBackend.h:40-53). Its debug block holds the reference
+// diagnostic counters that have a meaning under exact CPU semantics, computed from the final state:
+// ticks, scheduled requests (emissions from fires before the last tick) and in-window events,
+// processed synaptic events, fires, STDP weight changes, events per receptor, dendritic
+// integrations, post-spike plasticity scans, temporal observations / matches / learned patterns and
+// the per-tick maxima. Lock, drop and overflow counters stay 0: the step takes no locks and drops
+// nothing (a capacity overflow throws Error).
+template <class Result, class Connectome, class Id>
+Result run_resident_propagation(const Connectome& connectome, const std::vector<Id>& initially_active,
+                                std::uint32_t max_steps, const snn_options* opts = nullptr) {
+    Result result{};
+    if (connectome.neurons.empty() || initially_active.empty()) {
+        result.executed = true;
+        return result;
+    }
+    const auto start = std::chrono::steady_clock::now();
+    const PackedConnectome p = pack(connectome);
+    std::vector<std::uint32_t> active;
+    {
+        std::unordered_set<std::uint64_t> seen;
+        for (const auto id : initially_active) {
+            const auto it = p.index.find(static_cast<std::uint64_t>(id));
+            if (it != p.index.end() && seen.insert(static_cast<std::uint64_t>(id)).second) {
+                active.push_back(it->second);
+            }
+        }
+    }
+    if (active.empty()) {
+        result.executed = true;
+        return result;
+    }
+    const std::size_t n = p.threshold.size();
+    const std::size_t e = p.source.size();
+    snn_options o{};
+    if (opts) {
+        o = *opts;
+    }
+    o.track_pre_count = 1;
+    Network net(p.desc(), &o);
+    net.inject(active, std::vector<std::uint64_t>(active.size(), 0), std::vector<float>(active.size(), 1.0F));
+
+    // Fires before the last tick emit into the schedule; the reference counts those emissions as
+    // scheduled requests, so the spike counts are read once more just before the last tick.
+    std::vector<std::uint64_t> spikes_before_last(n, 0);
+    if (max_steps > 0) {
+        net.run_until(max_steps - 1);
+        check(snn_net_read_neurons(net.get(), nullptr, nullptr, nullptr, spikes_before_last.data(), nullptr,
+                                   nullptr, nullptr),
+              "snn_net_read_neurons");
+    }
+    net.run_until(max_steps);
+    const auto counts = net.counts();
+
+    result.executed = true;
+    result.delivered_spikes = counts.delivered;
+    result.fired_spikes = counts.fired;
+    result.final_synapse_weights.resize(e);
+    result.final_membrane_potentials.resize(n);
+    result.final_neuron_spike_counts.resize(n);
+    std::vector<std::uint64_t> t_match(n), t_learned(n), c_match(e), c_learned(e), pre_count(e), ltp(e), ltd(e);
+    check(snn_net_read_neurons(net.get(), result.final_membrane_potentials.data(), nullptr, nullptr,
+                               result.final_neuron_spike_counts.data(), nullptr, t_match.data(), t_learned.data()),
+          "snn_net_read_neurons");
+    check(snn_net_read_synapses(net.get(), result.final_synapse_weights.data(), pre_count.data(), nullptr,
+                                nullptr, nullptr, c_match.data(), c_learned.data()),
+          "snn_net_read_synapses");
+    check(snn_net_read_plasticity(net.get(), ltp.data(), ltd.data()), "snn_net_read_plasticity");
+    result.final_temporal_match_counts.assign(t_match.begin(), t_match.end());
+    result.final_temporal_learned_pattern_counts.resize(n);
+    for (std::size_t i = 0; i < n; ++i) {
+        result.final_temporal_learned_pattern_counts[i] = static_cast<std::uint32_t>(t_learned[i]);
+    }
+    result.final_synapse_code_match_counts.assign(c_match.begin(), c_match.end());
+    result.final_synapse_code_learned_pattern_counts.resize(e);
+    for (std::size_t j = 0; j < e; ++j) {
+        result.final_synapse_code_learned_pattern_counts[j] = static_cast<std::uint32_t>(c_learned[j]);
+    }
+
+    // Diagnostic counters.
+    const std::uint64_t ticks = static_cast<std::uint64_t>(max_steps) + 1;
+    std::vector<std::uint64_t> dpt(ticks), fpt(ticks);
+    check(snn_net_read_tick_counts(net.get(), 0, ticks, dpt.data(), fpt.data()), "snn_net_read_tick_counts");
+    const std::uint64_t synaptic = counts.delivered - active.size();
+    std::uint64_t requests = 0, scans = 0, receptor[4] = {0, 0, 0, 0}, stdp_ltp = 0, stdp_ltd = 0;
+    std::vector<std::uint64_t> emit(n, 0), indeg(n, 0);
+    for (std::size_t j = 0; j < e; ++j) {
+        emit[p.source[j]] += std::max<std::uint64_t>(1, p.code_ptr[j + 1] - p.code_ptr[j]);
+        indeg[p.target[j]] += 1;
+        receptor[p.receptor[j] & 3] += pre_count[j];
+        stdp_ltp += ltp[j];
+        stdp_ltd += ltd[j];
+    }
+    std::uint64_t t_matches = 0, t_learned_total = 0;
+    for (std::size_t i = 0; i < n; ++i) {
+        requests += spikes_before_last[i] * emit[i];
+        scans += result.final_neuron_spike_counts[i] * indeg[i];
+        t_matches += t_match[i];
+        t_learned_total += t_learned[i];
+    }
+    auto& d = result.debug;
+    d.ticks_processed = ticks;
+    d.scheduled_event_requests = requests;
+    d.scheduled_events = synaptic;
+    d.processed_events = synaptic;
+    d.fired_appends = counts.fired;
+    d.stdp_updates = stdp_ltp + stdp_ltd;
+    d.stdp_ltp = stdp_ltp;
+    d.stdp_ltd = stdp_ltd;
+    d.receptor_ampa_events = receptor[0];
+    d.receptor_nmda_events = receptor[1];
+    d.receptor_gaba_a_events = receptor[2];
+    d.receptor_gaba_b_events = receptor[3];
+    d.dendritic_integrations = synaptic;
+    d.post_plasticity_scans = scans;
+    d.temporal_observations = counts.delivered + synaptic;
+    d.temporal_patterns_learned = t_learned_total;
+    d.temporal_matches = t_matches;
+    d.max_scheduled_events_per_tick = *std::max_element(dpt.begin(), dpt.end());
+    d.max_fired_neurons_per_tick = *std::max_element(fpt.begin(), fpt.end());
+    result.elapsed_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    return result;
+}
+
+}  // namespace snncuda_hip
+
+#endif

@@ -49,6 +49,7 @@ def lib() -> ctypes.CDLL:
         L.snno_read_fires.argtypes = [P, P, P]
         L.snno_read_neurons.argtypes = [P] * 8
         L.snno_read_synapses.argtypes = [P] * 8
+        L.snno_read_ltp.argtypes = [P, P]
         L.snno_run_file.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
         L.snno_run_file.restype = ctypes.c_int
         _lib = L
@@ -243,4 +244,6 @@ class Oracle:
                    c_learned=np.zeros(e, np.uint64))
         lib().snno_read_synapses(self.h, *[_p(out[k]) for k in ("weight", "pre_count", "upd_count", "last_pre",
                                                                   "last_post", "c_match", "c_learned")])
+        out["ltp"] = np.zeros(e, np.uint64)
+        lib().snno_read_ltp(self.h, _p(out["ltp"]))
         return out

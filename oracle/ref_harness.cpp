@@ -4,9 +4,6 @@
 // golden fixtures under tests/golden/ and to check oracle/snn_oracle.c.
 //
 // Usage: ref_harness <net.bin> <out.bin>
-//
-// The Connectome is built directly (not through NetworkIR) with ids i+1 / j+1, which is what
-// ConnectomeBuilder produces for the same neuron and synapse order.
 
 #include "snncuda/declarative/Connectome.h"
 #include "snncuda/runtime/NetworkPropagator.h"
@@ -21,26 +18,9 @@
 #include <string>
 #include <vector>
 
-extern "C" {
-#include "netfile.h"
-}
+#include "net_connectome.hpp"
 
 namespace {
-
-struct Reader {
-    FILE* f;
-    template <typename T>
-    std::vector<T> vec(std::size_t n) {
-        std::vector<T> v(n);
-        if (n && std::fread(v.data(), sizeof(T), n, f) != n) throw std::runtime_error("short read");
-        return v;
-    }
-    std::uint64_t u64() {
-        std::uint64_t x;
-        if (std::fread(&x, 8, 1, f) != 1) throw std::runtime_error("short read");
-        return x;
-    }
-};
 
 struct Writer {
     FILE* f;
@@ -70,56 +50,11 @@ int main(int argc, char** argv) {
     }
     const bool time_only = argc == 4;
     try {
-        FILE* in = std::fopen(argv[1], "rb");
-        if (!in) throw std::runtime_error("cannot open input");
-        char magic[8];
-        if (std::fread(magic, 1, 8, in) != 8 || std::memcmp(magic, SNN_NET_MAGIC, 8) != 0)
-            throw std::runtime_error("bad magic");
-        Reader r{in};
-        const auto n = r.u64();
-        const auto e = r.u64();
-        const auto ncodes = r.u64();
-        const auto ncmd = r.u64();
-        auto thr = r.vec<float>(n);
-        auto win = r.vec<std::uint64_t>(n);
-        auto sim = r.vec<float>(n);
-        auto mref = r.vec<std::uint64_t>(n);
-        auto src = r.vec<std::uint32_t>(e);
-        auto tgt = r.vec<std::uint32_t>(e);
-        auto w = r.vec<float>(e);
-        auto mw = r.vec<float>(e);
-        auto dl = r.vec<std::uint32_t>(e);
-        auto comp = r.vec<std::uint8_t>(e);
-        auto rec = r.vec<std::uint8_t>(e);
-        auto pl = r.vec<std::uint8_t>(e);
-        auto cptr = r.vec<std::uint64_t>(e + 1);
-        auto codes = r.vec<std::uint32_t>(ncodes);
-        auto cmds = r.vec<snn_command>(ncmd);
-        std::fclose(in);
-
-        Connectome c;
-        c.neurons.resize(n);
-        for (std::uint64_t i = 0; i < n; ++i) {
-            c.neurons[i].id = i + 1;
-            c.neurons[i].params.threshold = thr[i];
-            c.neurons[i].params.pattern_window_ticks = win[i];
-            c.neurons[i].params.similarity_threshold = sim[i];
-            c.neurons[i].params.max_reference_patterns = mref[i];
-        }
-        c.synapses.resize(e);
-        for (std::uint64_t j = 0; j < e; ++j) {
-            auto& s = c.synapses[j];
-            s.id = j + 1;
-            s.source = src[j] + 1;
-            s.target = tgt[j] + 1;
-            s.weight = w[j];
-            s.max_weight = mw[j];
-            s.delay_ticks = dl[j];
-            s.spike_code_offsets.assign(codes.begin() + cptr[j], codes.begin() + cptr[j + 1]);
-            s.compartment = static_cast<snncuda::snn::DendriticCompartment>(comp[j]);
-            s.receptor = static_cast<snncuda::snn::ReceptorType>(rec[j]);
-            s.plasticity_enabled = pl[j] != 0;
-        }
+        const netfile::Net t = netfile::read(argv[1]);
+        const auto n = t.n;
+        const auto e = t.e;
+        const auto& cmds = t.cmds;
+        const Connectome c = netfile::connectome(t);
 
         snncuda::runtime::MemoryNeuronStateStore store;
         snncuda::runtime::NetworkPropagator prop(c, store, n ? n : 1, 4096);

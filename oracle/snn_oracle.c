@@ -108,7 +108,7 @@ typedef struct {
     const uint32_t* codes; /* points into snno.codes; ncodes >= 1 */
     uint8_t comp, receptor, plastic;
     int64_t last_pre, last_post; /* -1: std::nullopt */
-    uint64_t pre_count, upd_count;
+    uint64_t pre_count, upd_count, ltp_count;
     float last_delta;
     uint64_t c_start;
     u32vec c_offsets;
@@ -256,13 +256,19 @@ static float stdp_delta(uint64_t pre, uint64_t post) {
     if (dt < 0.0f) return -0.012f * expf(dt / 20.0f);
     return 0.0f;
 }
+/* A changed update counts as LTP when its delta is positive, else as LTD: the split of the
+ * reference's stdp_ltp / stdp_ltd diagnostic counters (src/backends/cuda/CudaBackend.cu:133-139). */
 static void stdp_apply(synapse_t* s, uint64_t pre, uint64_t post) {
     const float before = s->weight;
     const float maximum = (s->max_weight < 2.0f) ? s->max_weight : 2.0f;
-    const float v = s->weight + stdp_delta(pre, post);
+    const float delta = stdp_delta(pre, post);
+    const float v = s->weight + delta;
     s->weight = (v < 0.0f) ? 0.0f : (maximum < v) ? maximum : v;
     s->last_delta = s->weight - before;
-    if (s->last_delta != 0.0f) s->upd_count++;
+    if (s->last_delta != 0.0f) {
+        s->upd_count++;
+        if (delta > 0.0f) s->ltp_count++;
+    }
 }
 
 /* decay_compartments (src/runtime/SynapseDendriteProcessor.cpp:214-229) */
@@ -587,6 +593,10 @@ void snno_read_synapses(const snno* o, float* weight, uint64_t* pre_count, uint6
         if (code_match) code_match[j] = s->c_match;
         if (code_learned) code_learned[j] = s->c_learned.n;
     }
+}
+
+void snno_read_ltp(const snno* o, uint64_t* ltp_count) {
+    for (uint64_t j = 0; j < o->n_syn; ++j) ltp_count[j] = o->syn[j].ltp_count;
 }
 
 /* ---------------------------------------------------------------- file driver */

@@ -52,6 +52,8 @@ void snno_read_neurons(const snno* o, float* membrane, float* currents, float* t
 void snno_read_synapses(const snno* o, float* weight, uint64_t* pre_count, uint64_t* upd_count,
                         int64_t* last_pre, int64_t* last_post, uint64_t* code_match,
                         uint64_t* code_learned);
+/* Changed STDP updates with a positive delta, per synapse (the rest of upd_count had delta <= 0). */
+void snno_read_ltp(const snno* o, uint64_t* ltp_count);
 
 /* Run a SNNNET01 file's commands and write a SNNOUT01 result file (same layout as the reference
  * harness writes). Returns 0 on success. If seconds is non-NULL it receives the time spent inside

@@ -94,11 +94,13 @@ def _load() -> ctypes.CDLL:
     L.snn_net_read_raster.argtypes = [P, P, P, U64]
     L.snn_net_read_neurons.argtypes = [P] * 8
     L.snn_net_read_synapses.argtypes = [P] * 8
+    L.snn_net_read_plasticity.argtypes = [P] * 3
     L.snn_net_set_profiling.argtypes = [P, I]
     L.snn_net_kernel_times.argtypes = [P, P, P, I, ctypes.c_char_p, U64]
     for name in ("snn_net_create", "snn_net_create_ei", "snn_net_destroy", "snn_net_inject", "snn_net_run_until", "snn_net_counts",
                  "snn_net_read_tick_counts", "snn_net_read_raster", "snn_net_read_neurons",
-                 "snn_net_read_synapses", "snn_net_set_profiling", "snn_net_kernel_times"):
+                 "snn_net_read_synapses", "snn_net_read_plasticity", "snn_net_set_profiling",
+                 "snn_net_kernel_times"):
         getattr(L, name).restype = I
     return L
 
@@ -267,6 +269,13 @@ class Network:
         _check(lib().snn_net_read_synapses(self._h, _p(o["weight"]), _p(o.get("pre_count")), _p(o["upd_count"]),
                                            _p(o["last_pre"]), _p(o["last_post"]), _p(o["c_match"]),
                                            _p(o["c_learned"])))
+        return o
+
+    def plasticity(self) -> Dict[str, np.ndarray]:
+        """Changed STDP updates per synapse (connectome order) split into delta > 0 (ltp) and the
+        rest (ltd); ltp + ltd == synapses()["upd_count"]."""
+        o = dict(ltp=np.zeros(self.e, np.uint64), ltd=np.zeros(self.e, np.uint64))
+        _check(lib().snn_net_read_plasticity(self._h, _p(o["ltp"]), _p(o["ltd"])))
         return o
 
     def set_profiling(self, on: bool) -> None:

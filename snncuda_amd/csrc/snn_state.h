@@ -404,8 +404,22 @@ struct StdpRec {           // 16 B
 
 // StdpRule::apply: clamp(w + delta, 0, min(2, max_w)), std::clamp written out as libstdc++ does.
 // ltp[a] = 0.01f*expf(-a/20f), ltd[a] = -0.012f*expf(-a/20f) for a < kStdpTable, 0 beyond.
-SNN_HD bool stdp_apply(StdpRec& r, uint32_t pre, uint32_t post, const float* ltp, const float* ltd,
-                       const float* wmax) {
+// Returns 0 when the weight is unchanged, kStdpLtp when it changed with delta > 0, kStdpLtd when
+// it changed otherwise (the reference's split of changed updates into LTP and LTD,
+// src/backends/cuda/CudaBackend.cu:133-139).
+constexpr uint32_t kStdpLtd = 1u;
+constexpr uint32_t kStdpLtp = 1u | (1u << 16);
+
+// Changed-update tally of one synapse over several applications.
+struct StdpTally {
+    uint32_t n = 0, ltp = 0;
+    SNN_HD void add(uint32_t c) {
+        n += c & 1u;
+        ltp += c >> 16;
+    }
+};
+SNN_HD uint32_t stdp_apply(StdpRec& r, uint32_t pre, uint32_t post, const float* ltp, const float* ltd,
+                           const float* wmax) {
     float delta = 0.0f;
     if (post > pre) {
         const uint32_t a = post - pre;
@@ -418,7 +432,8 @@ SNN_HD bool stdp_apply(StdpRec& r, uint32_t pre, uint32_t post, const float* ltp
     const float v = r.weight + delta;
     const float hi = wmax[r.aux & 0xFFu];
     r.weight = (v < 0.0f) ? 0.0f : (hi < v) ? hi : v;
-    return (r.weight - before) != 0.0f;
+    if ((r.weight - before) == 0.0f) return 0u;
+    return delta > 0.0f ? kStdpLtp : kStdpLtd;
 }
 
 // ------------------------------------------------------------------ decay

@@ -77,7 +77,8 @@ struct Step {
     const uint32_t* tpos;
     const uint64_t* in_ptr;
     StdpRec* stdp;
-    uint32_t* upd_count;
+    uint32_t* upd_count;   // changed STDP updates per plastic synapse (plasticity_update_count)
+    uint32_t* ltp_count;   // of which with delta > 0
     const float* wmax;           // wmax table indexed by StdpRec::aux bits 0-7
     const NeuronParams* np;
     uint32_t tp_uniform;         // every neuron has the same window / similarity / max_refs
@@ -296,15 +297,20 @@ SNN_HD float u2f(uint32_t u) {
 
 // Bring a record up to `fires` post-synaptic updates of tick T (apply_post_spike_plasticity,
 // SynapseDendriteProcessor.cpp:138-173, once per fire of the target at T).
-SNN_HD uint32_t post_catch_up(const Step& S, StdpRec& r, uint32_t T, uint32_t fires) {
+SNN_HD StdpTally post_catch_up(const Step& S, StdpRec& r, uint32_t T, uint32_t fires) {
     uint32_t applied = r.last_post == T ? (r.aux >> 8) : 0u;
-    uint32_t changed = 0;
+    StdpTally changed;
     for (; applied < fires; ++applied) {
         r.last_post = T;
-        if (r.last_pre != kNone && stdp_apply(r, r.last_pre, T, S.ltp, S.ltd, S.wmax)) ++changed;
+        if (r.last_pre != kNone) changed.add(stdp_apply(r, r.last_pre, T, S.ltp, S.ltd, S.wmax));
     }
     if (r.last_post == T) r.aux = (r.aux & 0xFFu) | (applied << 8);
     return changed;
+}
+
+SNN_HD void tally_store(const Step& S, uint64_t t, StdpTally c) {
+    if (c.n) S.upd_count[t] += c.n;
+    if (c.ltp) S.ltp_count[t] += c.ltp;
 }
 
 // The sequential fold of target x's k events at tick T (ev is the target's own array, in any
@@ -332,14 +338,14 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
             if (STDP && e.tpos != kNone) {
                 StdpRec r = S.stdp[e.tpos];
                 // first the post-synaptic updates of this tick's earlier fires of x
-                uint32_t changed = post_catch_up(S, r, T, fires);
+                StdpTally changed = post_catch_up(S, r, T, fires);
                 // pre-synaptic side: process_synaptic_input (:95-126); the code offset is ignored
                 const uint32_t dl = S.delay[e.slot];
                 const uint32_t pre = T >= dl ? T - dl : T;
-                if (r.last_post != kNone && stdp_apply(r, pre, r.last_post, S.ltp, S.ltd, S.wmax)) ++changed;
+                if (r.last_post != kNone) changed.add(stdp_apply(r, pre, r.last_post, S.ltp, S.ltd, S.wmax));
                 r.last_pre = pre;
                 S.stdp[e.tpos] = r;
-                if (changed) S.upd_count[e.tpos] += changed;
+                tally_store(S, e.tpos, changed);
                 w = r.weight;
             }
             if (!CODE_HERE && e.slot != kNone) {
@@ -384,9 +390,9 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
                 const uint32_t t = ev[i].tpos;
                 if (t == kNone) continue;
                 StdpRec r = S.stdp[t];
-                const uint32_t changed = post_catch_up(S, r, T, fires);
+                const StdpTally changed = post_catch_up(S, r, T, fires);
                 S.stdp[t] = r;
-                if (changed) S.upd_count[t] += changed;
+                tally_store(S, t, changed);
             }
         }
     }
@@ -399,9 +405,9 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
 SNN_HD void post_scan_record(const Step& S, uint64_t t, uint32_t T, uint32_t m) {
     StdpRec r = S.stdp[t];
     if (r.last_post == T && (r.aux >> 8) == m) return;  // done inside the fold
-    const uint32_t changed = post_catch_up(S, r, T, m);
+    const StdpTally changed = post_catch_up(S, r, T, m);
     S.stdp[t] = r;
-    if (changed) S.upd_count[t] += changed;
+    tally_store(S, t, changed);
 }
 
 }  // namespace snn

@@ -1092,6 +1092,7 @@ int snn_net_create(const snn_net_desc* desc, const snn_options* opts, snn_net** 
             S.in_ptr = dev_upload(own, h.in_ptr);
             S.stdp = dev_upload(own, h.stdp);
             S.upd_count = dev_alloc<uint32_t>(own, h.stdp.size());
+            S.ltp_count = dev_alloc<uint32_t>(own, h.stdp.size());
             S.wmax = dev_upload(own, h.wmax);
             net->tpos_conn_dev = dev_upload(own, h.tpos_conn);
             net->n_tpos = h.stdp.size();
@@ -1193,6 +1194,7 @@ int snn_net_create_ei(const snn_ei_desc* g, const snn_options* opts, snn_net** o
             S.tpos = tpos;
             S.stdp = stdp;
             S.upd_count = dev_alloc<uint32_t>(own, e);
+            S.ltp_count = dev_alloc<uint32_t>(own, e);
             std::vector<float> wm(1, 2.0f);  // max_weight 2.0 on every synapse: min(2, 2)
             S.wmax = dev_upload(own, wm);
             net->tpos_conn_dev = tconn;
@@ -1448,6 +1450,28 @@ int snn_net_read_synapses(snn_net* net, float* weight, uint64_t* pre_count, uint
             }
         }
         check(hipStreamSynchronize(net->stream), "read synapses");
+        return SNN_OK;
+    });
+}
+
+int snn_net_read_plasticity(snn_net* net, uint64_t* ltp_updates, uint64_t* ltd_updates) {
+    if (!net) return fail(SNN_EINVAL, "null handle");
+    return guarded([&]() -> int {
+        check(hipSetDevice(net->device), "hipSetDevice");
+        const uint64_t e = net->e;
+        if (ltp_updates) std::fill(ltp_updates, ltp_updates + e, 0);
+        if (ltd_updates) std::fill(ltd_updates, ltd_updates + e, 0);
+        if (net->n_tpos == 0 || !(ltp_updates || ltd_updates)) return SNN_OK;
+        const uint64_t nt = net->n_tpos;
+        std::vector<uint32_t> upd(nt), ltp(nt), conn(nt);
+        check(hipStreamSynchronize(net->stream), "sync");
+        check(hipMemcpy(upd.data(), net->S.upd_count, nt * 4, hipMemcpyDeviceToHost), "read stdp");
+        check(hipMemcpy(ltp.data(), net->S.ltp_count, nt * 4, hipMemcpyDeviceToHost), "read stdp");
+        check(hipMemcpy(conn.data(), net->tpos_conn_dev, nt * 4, hipMemcpyDeviceToHost), "read stdp");
+        for (uint64_t t = 0; t < nt; ++t) {
+            if (ltp_updates) ltp_updates[conn[t]] = ltp[t];
+            if (ltd_updates) ltd_updates[conn[t]] = upd[t] - ltp[t];
+        }
         return SNN_OK;
     });
 }

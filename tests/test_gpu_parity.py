@@ -50,6 +50,9 @@ def test_device_matches_oracle_other_seeds(seed):
             ds, rs = dev.synapses(), ref.synapses()
             for k in ("weight", "pre_count", "upd_count", "last_pre", "last_post", "c_match", "c_learned"):
                 assert np.array_equal(ds[k], rs[k].astype(ds[k].dtype)), (net.name, k)
+            pl = dev.plasticity()
+            assert np.array_equal(pl["ltp"], rs["ltp"]), (net.name, "ltp")
+            assert np.array_equal(pl["ltp"] + pl["ltd"], ds["upd_count"]), (net.name, "ltp + ltd")
 
 
 def test_capacity_overflow_raises():

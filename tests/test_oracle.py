@@ -78,13 +78,16 @@ def test_kat_ltp_closed_form():
     s = o.synapses()
     assert s["upd_count"][0] == 1
     assert abs(float(s["weight"][0]) - (0.5 + 0.01 * math.exp(-1 / 20))) < 1e-7
+    assert s["ltp"][0] == 1  # post after pre: the one change is potentiation
 
 
 def test_kat_ltd_direction():
     # synapse_dendrite_tests.cpp:212-236: pre at 5 after post at 0 -> depression by 0.012*exp(-5/20)
     o = _run(K.kat_ltd())
-    w = float(o.synapses()["weight"][0])
+    s = o.synapses()
+    w = float(s["weight"][0])
     assert w < 0.5 and abs(w - (0.5 - 0.012 * math.exp(-5 / 20))) < 1e-7
+    assert s["upd_count"][0] == 1 and s["ltp"][0] == 0  # pre after post: depression only
     assert o.neurons()["spike_count"][1] == 1
 
 
@@ -113,7 +116,11 @@ def test_kat_learning_experiments():
     assert o.neurons()["t_match"][net.info["probe"]] >= 1
     net = K.learning_consistency()
     o = _run(net)
-    assert o.synapses()["weight"][1] > 0.5 and o.neurons()["t_match"][net.info["probe"]] >= 1
+    s = o.synapses()
+    assert s["weight"][1] > 0.5 and o.neurons()["t_match"][net.info["probe"]] >= 1
+    # cuda_learning_consistency_experiment.cpp:297: net causal LTP
+    ltp = int(s["ltp"].sum())
+    assert ltp > int(s["upd_count"].sum()) - ltp
 
 
 def test_delivery_ticks_follow_fire_tick_plus_delay_plus_offset():
