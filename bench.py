@@ -54,6 +54,13 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-neurons", type=int, default=20_000)
     ap.add_argument("--cpu-ticks", type=int, default=200)
+    ap.add_argument("--form", choices=("plain", "batched"), default="plain",
+                    help="plain: the 1M-neuron step (default); batched: 512 uneven networks in one launch "
+                         "sequence vs looping the plain step over the same networks")
+    ap.add_argument("--nets", type=int, default=512, help="batched form: networks")
+    ap.add_argument("--batch-ticks", type=int, default=2000, help="batched form: ticks per network")
+    ap.add_argument("--loop-nets", type=int, default=0,
+                    help="batched form: networks in the looped baseline (0: all of them)")
     return ap.parse_args()
 
 
@@ -83,12 +90,102 @@ def cpu_baseline(args):
                       f"{args.cpu_ticks} ticks incl. ignition, {ev} events in {dt:.1f} s, 1 thread"}
 
 
+def batched(args):
+    """512 independent networks of the batched workload (log-uniform sizes 500..50,000, fan-out
+    U{50..500}, 2 code offsets, 100 Poisson sources each, STDP on, threshold 3.0), generated on the
+    device, stepped together for --batch-ticks ticks under the online autotuner; against looping the
+    plain step over the same networks, once with stream launches and once with HIP graph replay.
+    value = networks/s of the batched run (networks x their full run / wall time)."""
+    import numpy as np
+    import snncuda_amd as sa
+    from snncuda_amd import workloads as W
+
+    T = args.batch_ticks
+    specs, cmds = zip(*[W.config3_spec(s, ticks=T) for s in range(args.nets)])
+    n_total = sum(sp["n"] + sp["n_sources"] for sp in specs)
+    e_total = sum(int(sp["fanout"].sum()) + sp["n_sources"] * sp["source_fanout"] for sp in specs)
+    n_inj = sum(sum(1 for c in cm if c[0] == 0) for cm in cmds)
+    print(f"batched: {args.nets} networks, {n_total} neurons, {e_total} synapses, {T} ticks", file=sys.stderr, flush=True)
+    t_gen = time.perf_counter()
+    with sa.Batch(specs=list(specs), track_pre_count=False, max_events_per_tick=1 << 24,
+                  fire_log_capacity=1 << 22, raster_capacity=1 << 22) as b:
+        t_gen = time.perf_counter() - t_gen
+        u = b.union
+        merged, owner = W.merge_commands(cmds, b.neuron_off)
+        inj = np.array([c for c in merged if c[0] == 0], dtype=np.float64)
+        own = np.asarray(owner, np.uint32)
+        b.inject(own, inj[:, 1].astype(np.int64) - b.neuron_off[own].astype(np.int64), inj[:, 2].astype(np.uint64),
+                 inj[:, 3].astype(np.float32))
+        u.autotune(16)
+        t0 = time.perf_counter()
+        b.run_until(T - 1)
+        t_batch = time.perf_counter() - t0
+        delivered, fired, _ = u.counts()
+        schedule = u.schedule()
+        tune = u.tune_log()
+    print(f"batched run {t_batch:.3f} s ({args.nets / t_batch:.1f} networks/s); schedule {schedule}",
+          file=sys.stderr, flush=True)
+
+    loop_nets = args.loop_nets or args.nets
+    loops = {}
+    for mode, graph in (("stream", 0), ("graph", 32)):
+        total = 0.0
+        for k in range(loop_nets):
+            with sa.Network(spec=specs[k], track_pre_count=False, max_events_per_tick=1 << 22,
+                            fire_log_capacity=1 << 20, raster_capacity=1 << 16) as net:
+                net.set_schedule(graph_ticks=graph)
+                net.run_commands([c for c in cmds[k] if c[0] == 0])
+                t0 = time.perf_counter()
+                net.run_until(T - 1)
+                total += time.perf_counter() - t0
+        loops[mode] = total * args.nets / loop_nets
+        print(f"loop ({mode}) {loops[mode]:.3f} s for {args.nets} networks", file=sys.stderr, flush=True)
+    best_loop = min(loops.values())
+    value = args.nets / t_batch
+    return {
+        "metric": "networks/s (512 uneven networks, batched vs looped plain step)",
+        "value": value,
+        "unit": "networks/s",
+        "n_gpus": 1,
+        "steps": T,
+        "warmup": 0,
+        "ms_per_step": t_batch / T * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded generator, built on the device)",
+        "config": {
+            "workload": f"{args.nets} E/I networks, sizes log-uniform in [500, 50000], fan-out U{{50..500}}, "
+                        "80/20 E/I, delays U{1..20}, 2 compartments, 2 code offsets U{0..7}, 100 Poisson "
+                        "sources p=0.01 per network, STDP on, threshold 3.0",
+            "neurons": n_total,
+            "synapses": e_total,
+            "ticks": T,
+            "events_per_tick": (delivered - n_inj) / T,
+            "fires_per_tick": fired / T,
+            "parallelism": "1 GPU",
+        },
+        "speedup_vs_loop": best_loop / t_batch,
+        "loop_seconds": loops,
+        "loop_networks_timed": loop_nets,
+        "batched_seconds": t_batch,
+        "generate_seconds": t_gen,
+        "events_per_s": (delivered - n_inj) / t_batch,
+        "schedule": schedule,
+        "autotune": tune.strip().split("\n"),
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the step shards by target neuron "
               "across GPUs by design, but no multi-GPU path is built or measured", file=sys.stderr)
         sys.exit(2)
+    if args.form == "batched":
+        print(json.dumps(batched(args)), flush=True)
+        return
     import numpy as np
     import snncuda_amd as sa
     from snncuda_amd import workloads as W

@@ -80,7 +80,7 @@ int snn_net_destroy(snn_net* net);
  * (the network the reference would receive as a Connectome; used for benchmarks too large to pack
  * on the host). Network neurons [0, n) with fan-out fanout[i], then n_sources input neurons with
  * source_fanout each; the same counter hash as snncuda_amd/workloads.py's ei_network, so the result
- * equals snn_net_create on that function's arrays. One code offset {0} per synapse. */
+ * equals snn_net_create on that function's arrays. */
 typedef struct snn_ei_desc {
     uint64_t n;
     uint64_t n_sources;
@@ -99,6 +99,8 @@ typedef struct snn_ei_desc {
     uint32_t window;
     float sim_thr;
     uint32_t max_refs;
+    uint32_t n_codes;           /* code offsets per synapse, 1..8 (0: one code {0}) */
+    uint32_t code_max;          /* offsets drawn from U{0..code_max} (0: every offset is 0); <= 63 */
 } snn_ei_desc;
 int snn_net_create_ei(const snn_ei_desc* desc, const snn_options* opts, snn_net** out);
 
@@ -137,6 +139,51 @@ int snn_net_read_synapses(snn_net* net, float* weight, uint64_t* pre_count, uint
  * them (reference src/backends/cuda/CudaBackend.cu:133-139). ltp + ltd equals update_count of
  * snn_net_read_synapses. Either pointer may be NULL. */
 int snn_net_read_plasticity(snn_net* net, uint64_t* ltp_updates, uint64_t* ltd_updates);
+
+/* Launch schedule: target partition size (2^partition_shift neurons, 10..12), workgroups of the
+ * partition count / scatter kernels, and ticks per HIP graph replay (0: one stream launch per kernel;
+ * n > 0: the launches of n ticks are captured once per snn_net_run_until call and replayed). Results
+ * are identical under every schedule. No counterpart in the reference (its launch shapes are fixed,
+ * src/backends/cuda/CudaBackend.cu:3672-3799). */
+typedef struct snn_schedule {
+    uint32_t partition_shift;
+    uint32_t scatter_workgroups;
+    uint32_t graph_ticks;
+    uint32_t reserved;
+} snn_schedule;
+int snn_net_get_schedule(snn_net* net, snn_schedule* out);
+int snn_net_set_schedule(snn_net* net, const snn_schedule* schedule);
+
+/* Online autotuner: the next snn_net_run_until call that is long enough spends its first ticks
+ * (ticks of the run itself, not extra work) on chunks of trial_ticks under candidate schedules, timed
+ * with HIP events on the handle's stream, and keeps the fastest for the rest of that call and later
+ * calls. trial_ticks 0 cancels a pending tune. snn_net_tune_log copies the trial table (text) and
+ * returns its length. */
+int snn_net_autotune(snn_net* net, uint32_t trial_ticks);
+int snn_net_tune_log(snn_net* net, char* buf, uint64_t cap);
+
+/* Batched form: independent networks of any sizes stepped together by one set of launches per tick
+ * (one graph replay per graph_ticks ticks). Replaces the reference's batched step
+ * (src/backends/cuda/CudaBackend.cu:837-1258, run_stateless_device_batch :2296-2578), which steps
+ * replicas of one connectome; here every network may differ. Network k's neurons and synapses occupy
+ * [neuron_off[k], neuron_off[k+1]) and [synapse_off[k], synapse_off[k+1]) of one union network, and
+ * its results equal a run of network k alone. snn_batch_network returns that union network
+ * (borrowed, valid until snn_batch_destroy) for the snn_net_read_* readers, schedules and the
+ * autotuner. Injects go through snn_batch_inject (network-local neuron indices). */
+typedef struct snn_batch snn_batch;
+int snn_batch_create(const snn_net_desc* nets, uint64_t n_nets, const snn_options* opts, snn_batch** out);
+/* No counterpart in the reference: generates n_nets seeded E/I networks (snn_net_create_ei's family,
+ * one descriptor each) on the device as a batch; equal to snn_batch_create on the host-built arrays. */
+int snn_batch_create_ei(const snn_ei_desc* nets, uint64_t n_nets, const snn_options* opts, snn_batch** out);
+int snn_batch_destroy(snn_batch* batch);
+int snn_batch_network(snn_batch* batch, snn_net** net);
+int snn_batch_offsets(snn_batch* batch, uint64_t* n_nets, uint64_t* neuron_off, uint64_t* synapse_off);
+int snn_batch_inject(snn_batch* batch, const uint32_t* net, const uint32_t* neuron, const uint64_t* tick,
+                     const float* weight, uint64_t n);
+int snn_batch_run_until(snn_batch* batch, uint64_t inclusive_tick);
+/* Per-network delivered (external injects included, as delivered_spike_count counts them) and fired
+ * totals, arrays of n_nets. delivered needs track_pre_count; either pointer may be NULL. */
+int snn_batch_net_counts(snn_batch* batch, uint64_t* delivered, uint64_t* fired);
 
 /* Mean duration (ms) of each step kernel over the last run, measured with HIP events on the
  * handle's stream when profiling is enabled (snn_net_set_profiling). names receives a

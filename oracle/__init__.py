@@ -50,6 +50,22 @@ def lib() -> ctypes.CDLL:
         L.snno_read_neurons.argtypes = [P] * 8
         L.snno_read_synapses.argtypes = [P] * 8
         L.snno_read_ltp.argtypes = [P, P]
+        L.snno_set_event_log.argtypes = [P, ctypes.c_int]
+        L.snno_event_log_size.argtypes = [P]
+        L.snno_event_log_size.restype = ctypes.c_uint64
+        L.snno_read_event_log.argtypes = [P] * 6
+        L.adjo_create.restype = P
+        L.adjo_create.argtypes = [ctypes.c_uint64] * 4 + [P] * 8
+        L.adjo_destroy.argtypes = [P]
+        L.adjo_event_count.argtypes = [P]
+        L.adjo_event_count.restype = ctypes.c_uint64
+        L.adjo_forward.argtypes = [P] * 5
+        L.adjo_loss.argtypes = [P] * 5
+        L.adjo_loss.restype = ctypes.c_double
+        L.adjo_loss_target.argtypes = [P, ctypes.c_uint64] + [P] * 4
+        L.adjo_loss_target.restype = ctypes.c_double
+        L.adjo_grad.argtypes = [P] * 8
+        L.adjo_grad.restype = ctypes.c_double
         L.snno_run_file.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
         L.snno_run_file.restype = ctypes.c_int
         _lib = L
@@ -177,7 +193,7 @@ def run_reference(net, patterns: bool = True) -> Results:
 class Oracle:
     """In-process oracle mirroring NetworkPropagator's inject / run_until / readers."""
 
-    def __init__(self, net):
+    def __init__(self, net, log_events: bool = False):
         L = lib()
         self._keep = [np.ascontiguousarray(a) for a in (
             net.threshold, net.window, net.sim_thr, net.max_refs, net.src, net.tgt, net.weight,
@@ -187,6 +203,17 @@ class Oracle:
         self.h = L.snno_create(self.n, _p(k[0]), _p(k[1]), _p(k[2]), _p(k[3]), self.e, *[_p(a) for a in k[4:]])
         if not self.h:
             raise ValueError("oracle rejected the network")
+        if log_events:
+            L.snno_set_event_log(self.h, 1)
+
+    def event_log(self) -> Dict[str, np.ndarray]:
+        """Every processed event in processing order: tick, target, synapse (2**32-1: external),
+        weight, fired (the target fired on this event)."""
+        m = int(lib().snno_event_log_size(self.h))
+        o = dict(tick=np.zeros(m, np.uint64), target=np.zeros(m, np.uint32), synapse=np.zeros(m, np.uint32),
+                 weight=np.zeros(m, np.float32), fired=np.zeros(m, np.uint8))
+        lib().snno_read_event_log(self.h, *[_p(o[k]) for k in ("tick", "target", "synapse", "weight", "fired")])
+        return o
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -247,3 +274,47 @@ class Oracle:
         out["ltp"] = np.zeros(e, np.uint64)
         lib().snno_read_ltp(self.h, _p(out["ltp"]))
         return out
+
+
+class AdjointOracle:
+    """FP64 fixed-spike adjoint (adjoint_oracle.c) over an Oracle run's event log."""
+
+    def __init__(self, net, ticks: int, log: Dict[str, np.ndarray]):
+        self.n, self.e, self.T = net.n_neurons, net.n_synapses, int(ticks)
+        self._keep = [np.ascontiguousarray(x) for x in (log["tick"], log["target"], log["synapse"], log["weight"],
+                                                          log["fired"], net.comp, net.receptor, net.tgt)]
+        k = self._keep
+        self.h = lib().adjo_create(self.n, self.e, self.T, int(k[0].shape[0]), *[_p(a) for a in k])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().adjo_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def _d(x):
+        return np.ascontiguousarray(x, dtype=np.float64)
+
+    def forward(self, w, alpha, tau) -> np.ndarray:
+        """Somatic traces [T, N]."""
+        out = np.zeros((self.T, self.n), np.float64)
+        w, alpha, tau = self._d(w), self._d(alpha), self._d(tau)
+        lib().adjo_forward(self.h, _p(w), _p(alpha), _p(tau), _p(out))
+        return out
+
+    def loss(self, w, alpha, tau, target) -> float:
+        w, alpha, tau, target = self._d(w), self._d(alpha), self._d(tau), self._d(target)
+        return float(lib().adjo_loss(self.h, _p(w), _p(alpha), _p(tau), _p(target)))
+
+    def loss_target(self, i, w, alpha, tau, target) -> float:
+        w, alpha, tau, target = self._d(w), self._d(alpha), self._d(tau), self._d(target)
+        return float(lib().adjo_loss_target(self.h, int(i), _p(w), _p(alpha), _p(tau), _p(target)))
+
+    def grad(self, w, alpha, tau, target):
+        """(loss, d/dw [E], d/dalpha [4], d/dtau [4])."""
+        w, alpha, tau, target = self._d(w), self._d(alpha), self._d(tau), self._d(target)
+        gw = np.zeros(self.e, np.float64)
+        ga = np.zeros(4, np.float64)
+        gt = np.zeros(4, np.float64)
+        loss = lib().adjo_grad(self.h, _p(w), _p(alpha), _p(tau), _p(target), _p(gw), _p(ga), _p(gt))
+        return float(loss), gw, ga, gt

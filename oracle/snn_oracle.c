@@ -154,6 +154,14 @@ struct snno {
     uint64_t* fire_tick;
     uint32_t* fire_neuron;
     size_t fires_cap;
+    /* optional log of every processed event, in processing order */
+    int log_on;
+    size_t log_n, log_cap;
+    uint64_t* log_tick;
+    uint32_t* log_target;
+    uint32_t* log_syn;
+    float* log_weight;
+    uint8_t* log_fired;
 };
 
 static void bucket_push(bucket_t* b, event_t ev) {
@@ -394,6 +402,23 @@ static void record_fire(snno* o, uint64_t tick, uint32_t neuron) {
     o->fire_neuron[o->fired] = neuron;
 }
 
+static void log_event(snno* o, uint64_t tick, const event_t* ev, int fired) {
+    if (o->log_n == o->log_cap) {
+        o->log_cap = o->log_cap ? o->log_cap * 2 : 4096;
+        o->log_tick = (uint64_t*)realloc(o->log_tick, o->log_cap * sizeof(uint64_t));
+        o->log_target = (uint32_t*)realloc(o->log_target, o->log_cap * sizeof(uint32_t));
+        o->log_syn = (uint32_t*)realloc(o->log_syn, o->log_cap * sizeof(uint32_t));
+        o->log_weight = (float*)realloc(o->log_weight, o->log_cap * sizeof(float));
+        o->log_fired = (uint8_t*)realloc(o->log_fired, o->log_cap);
+    }
+    o->log_tick[o->log_n] = tick;
+    o->log_target[o->log_n] = ev->target;
+    o->log_syn[o->log_n] = ev->synapse;
+    o->log_weight[o->log_n] = ev->weight;
+    o->log_fired[o->log_n] = (uint8_t)fired;
+    o->log_n++;
+}
+
 /* NetworkPropagator::process_tick (src/runtime/NetworkPropagator.cpp:78-132) */
 static void process_tick(snno* o, uint64_t tick) {
     const size_t nd = pop_due(o, tick);
@@ -409,6 +434,7 @@ static void process_tick(snno* o, uint64_t tick) {
         } else {
             fired = process_synaptic(t, &o->syn[ev.synapse], tick);
         }
+        if (o->log_on) log_event(o, tick, &ev, fired);
         if (!fired) continue;
         record_fire(o, tick, ev.target);
         o->fired++;
@@ -525,6 +551,11 @@ void snno_destroy(snno* o) {
     free(o->out_idx);
     free(o->in_idx);
     free(o->due);
+    free(o->log_tick);
+    free(o->log_target);
+    free(o->log_syn);
+    free(o->log_weight);
+    free(o->log_fired);
     free(o->tick_delivered);
     free(o->tick_fired);
     free(o->fire_tick);
@@ -592,6 +623,21 @@ void snno_read_synapses(const snno* o, float* weight, uint64_t* pre_count, uint6
         if (last_post) last_post[j] = s->last_post;
         if (code_match) code_match[j] = s->c_match;
         if (code_learned) code_learned[j] = s->c_learned.n;
+    }
+}
+
+void snno_set_event_log(snno* o, int on) { o->log_on = on; }
+
+uint64_t snno_event_log_size(const snno* o) { return o->log_n; }
+
+void snno_read_event_log(const snno* o, uint64_t* tick, uint32_t* target, uint32_t* synapse, float* weight,
+                         uint8_t* fired) {
+    for (size_t i = 0; i < o->log_n; ++i) {
+        tick[i] = o->log_tick[i];
+        target[i] = o->log_target[i];
+        synapse[i] = o->log_syn[i];
+        weight[i] = o->log_weight[i];
+        fired[i] = o->log_fired[i];
     }
 }
 

@@ -52,6 +52,13 @@ void snno_read_neurons(const snno* o, float* membrane, float* currents, float* t
 void snno_read_synapses(const snno* o, float* weight, uint64_t* pre_count, uint64_t* upd_count,
                         int64_t* last_pre, int64_t* last_post, uint64_t* code_match,
                         uint64_t* code_learned);
+/* Log of every processed event in processing order (off by default; switch on before running):
+ * tick, target, synapse (UINT32_MAX for an external inject), the event's weight and whether the
+ * target fired on it. */
+void snno_set_event_log(snno* o, int on);
+uint64_t snno_event_log_size(const snno* o);
+void snno_read_event_log(const snno* o, uint64_t* tick, uint32_t* target, uint32_t* synapse, float* weight,
+                         uint8_t* fired);
 /* Changed STDP updates with a positive delta, per synapse (the rest of upd_count had delta <= 0). */
 void snno_read_ltp(const snno* o, uint64_t* ltp_count);
 

@@ -64,7 +64,13 @@ class _EiDesc(ctypes.Structure):
                 ("w_exc", ctypes.c_float), ("w_inh", ctypes.c_float), ("delay_max", ctypes.c_uint32),
                 ("n_comps", ctypes.c_uint32), ("comps", ctypes.c_uint8 * 4), ("plastic", ctypes.c_int32),
                 ("threshold", ctypes.c_float), ("source_threshold", ctypes.c_float), ("window", ctypes.c_uint32),
-                ("sim_thr", ctypes.c_float), ("max_refs", ctypes.c_uint32)]
+                ("sim_thr", ctypes.c_float), ("max_refs", ctypes.c_uint32), ("n_codes", ctypes.c_uint32),
+                ("code_max", ctypes.c_uint32)]
+
+
+class Schedule(ctypes.Structure):
+    _fields_ = [("partition_shift", ctypes.c_uint32), ("scatter_workgroups", ctypes.c_uint32),
+                ("graph_ticks", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 def library_path() -> pathlib.Path:
@@ -95,12 +101,27 @@ def _load() -> ctypes.CDLL:
     L.snn_net_read_neurons.argtypes = [P] * 8
     L.snn_net_read_synapses.argtypes = [P] * 8
     L.snn_net_read_plasticity.argtypes = [P] * 3
+    L.snn_net_get_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
+    L.snn_net_set_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
+    L.snn_net_autotune.argtypes = [P, ctypes.c_uint32]
+    L.snn_net_tune_log.argtypes = [P, ctypes.c_char_p, U64]
+    L.snn_batch_create.argtypes = [P, U64, ctypes.POINTER(Options), ctypes.POINTER(P)]
+    L.snn_batch_create_ei.argtypes = [P, U64, ctypes.POINTER(Options), ctypes.POINTER(P)]
+    L.snn_batch_destroy.argtypes = [P]
+    L.snn_batch_network.argtypes = [P, ctypes.POINTER(P)]
+    L.snn_batch_offsets.argtypes = [P, P, P, P]
+    L.snn_batch_inject.argtypes = [P, P, P, P, P, U64]
+    L.snn_batch_run_until.argtypes = [P, U64]
+    L.snn_batch_net_counts.argtypes = [P, P, P]
     L.snn_net_set_profiling.argtypes = [P, I]
     L.snn_net_kernel_times.argtypes = [P, P, P, I, ctypes.c_char_p, U64]
     for name in ("snn_net_create", "snn_net_create_ei", "snn_net_destroy", "snn_net_inject", "snn_net_run_until", "snn_net_counts",
                  "snn_net_read_tick_counts", "snn_net_read_raster", "snn_net_read_neurons",
                  "snn_net_read_synapses", "snn_net_read_plasticity", "snn_net_set_profiling",
-                 "snn_net_kernel_times"):
+                 "snn_net_kernel_times", "snn_net_get_schedule", "snn_net_set_schedule", "snn_net_autotune",
+                 "snn_net_tune_log", "snn_batch_create", "snn_batch_create_ei", "snn_batch_destroy",
+                 "snn_batch_network",
+                 "snn_batch_offsets", "snn_batch_inject", "snn_batch_run_until", "snn_batch_net_counts"):
         getattr(L, name).restype = I
     return L
 
@@ -153,35 +174,36 @@ class Network:
         if spec is not None:
             self._create_from_spec(spec, o)
             return
-        self._arrays = [np.ascontiguousarray(a, dtype=dt) for a, dt in (
-            (net.threshold, np.float32), (net.window, np.uint64), (net.sim_thr, np.float32),
-            (net.max_refs, np.uint64), (net.src, np.uint32), (net.tgt, np.uint32), (net.weight, np.float32),
-            (net.max_weight, np.float32), (net.delay, np.uint32), (net.comp, np.uint8),
-            (net.receptor, np.uint8), (net.plastic, np.uint8), (net.code_ptr, np.uint64), (net.codes, np.uint32))]
-        a = self._arrays
-        self.n, self.e = int(a[0].shape[0]), int(a[4].shape[0])
-        d = _Desc(self.n, _p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), self.e, *[_p(x) for x in a[4:]])
+        d, keep = _desc(net)
+        self.n, self.e = int(d.n_neurons), int(d.n_synapses)
         h = ctypes.c_void_p()
         _check(L.snn_net_create(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
-        self._arrays = None  # the library copied everything
+        self._owned = True
+        del keep  # the library copied everything
+
+    @classmethod
+    def _borrowed(cls, handle, n: int, e: int, track_pre_count: bool) -> "Network":
+        """A view of a handle owned elsewhere (a batch's union network); close() leaves it alone."""
+        self = cls.__new__(cls)
+        self._h, self._owned = handle, False
+        self.n, self.e, self.track_pre_count = n, e, track_pre_count
+        return self
 
     def _create_from_spec(self, spec: dict, o: "Options") -> None:
         """Generate a workloads.ei_network on the device (snn_net_create_ei)."""
-        fan = np.ascontiguousarray(np.broadcast_to(np.asarray(spec["fanout"], np.uint32), (spec["n"],)))
-        comps = (ctypes.c_uint8 * 4)(*(list(spec["compartments"]) + [0] * 4)[:4])
-        d = _EiDesc(spec["n"], spec["n_sources"], _p(fan), spec["source_fanout"], spec["seed"], spec["n_exc"],
-                    spec["w_exc"], spec["w_inh"], spec["delay_max"], len(spec["compartments"]), comps,
-                    int(spec["plastic"]), spec["threshold"], 1.0, spec["window"], spec["sim_thr"], spec["max_refs"])
+        d, fan = _ei_desc(spec)
         h = ctypes.c_void_p()
         _check(lib().snn_net_create_ei(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
+        self._owned = True
         self.n = int(spec["n"] + spec["n_sources"])
         self.e = int(fan.astype(np.uint64).sum() + np.uint64(spec["n_sources"]) * np.uint64(spec["source_fanout"]))
 
     def close(self) -> None:
         if getattr(self, "_h", None):
-            lib().snn_net_destroy(self._h)
+            if getattr(self, "_owned", True):
+                lib().snn_net_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -278,6 +300,28 @@ class Network:
         _check(lib().snn_net_read_plasticity(self._h, _p(o["ltp"]), _p(o["ltd"])))
         return o
 
+    # launch schedule and autotuner (results are identical under every schedule)
+    def schedule(self) -> Dict[str, int]:
+        sc = Schedule()
+        _check(lib().snn_net_get_schedule(self._h, ctypes.byref(sc)))
+        return dict(partition_shift=sc.partition_shift, scatter_workgroups=sc.scatter_workgroups,
+                    graph_ticks=sc.graph_ticks)
+
+    def set_schedule(self, **kw) -> None:
+        cur = self.schedule()
+        cur.update({k: v for k, v in kw.items() if v is not None})
+        sc = Schedule(cur["partition_shift"], cur["scatter_workgroups"], cur["graph_ticks"], 0)
+        _check(lib().snn_net_set_schedule(self._h, ctypes.byref(sc)))
+
+    def autotune(self, trial_ticks: int = 32) -> None:
+        """Tune the schedule on the first ticks of the next long enough run_until call."""
+        _check(lib().snn_net_autotune(self._h, trial_ticks))
+
+    def tune_log(self) -> str:
+        buf = ctypes.create_string_buffer(1 << 16)
+        lib().snn_net_tune_log(self._h, buf, 1 << 16)
+        return buf.value.decode()
+
     def set_profiling(self, on: bool) -> None:
         _check(lib().snn_net_set_profiling(self._h, int(on)))
 
@@ -294,3 +338,136 @@ class Network:
 
 __all__ = ["SNNCudaLibraryError", "SNNError", "CapacityError", "Network", "Options", "version", "cuda_available",
            "device_count", "lib", "library_path"]
+
+
+def _ei_desc(spec: dict):
+    """snn_ei_desc of a workloads.ei_network spec; returns (desc, fan-out array to keep alive)."""
+    fan = np.ascontiguousarray(np.broadcast_to(np.asarray(spec["fanout"], np.uint32), (spec["n"],)))
+    comps = (ctypes.c_uint8 * 4)(*(list(spec["compartments"]) + [0] * 4)[:4])
+    d = _EiDesc(spec["n"], spec["n_sources"], _p(fan), spec["source_fanout"], spec["seed"], spec["n_exc"],
+                spec["w_exc"], spec["w_inh"], spec["delay_max"], len(spec["compartments"]), comps,
+                int(spec["plastic"]), spec["threshold"], 1.0, spec["window"], spec["sim_thr"], spec["max_refs"],
+                int(spec.get("n_codes", 1)), int(spec.get("code_max", 0)))
+    return d, fan
+
+
+def _desc(net):
+    """snn_net_desc over C-contiguous copies of a workloads.Net's arrays; returns (desc, arrays to keep
+    alive while the desc is used)."""
+    a = [np.ascontiguousarray(x, dtype=dt) for x, dt in (
+        (net.threshold, np.float32), (net.window, np.uint64), (net.sim_thr, np.float32),
+        (net.max_refs, np.uint64), (net.src, np.uint32), (net.tgt, np.uint32), (net.weight, np.float32),
+        (net.max_weight, np.float32), (net.delay, np.uint32), (net.comp, np.uint8),
+        (net.receptor, np.uint8), (net.plastic, np.uint8), (net.code_ptr, np.uint64), (net.codes, np.uint32))]
+    d = _Desc(int(a[0].shape[0]), _p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), int(a[4].shape[0]), *[_p(x) for x in a[4:]])
+    return d, a
+
+
+class Batch:
+    """Independent networks stepped together (snn_batch_*): one set of launches per tick for all of
+    them. ``union`` is the union network (a borrowed ``Network``) for state readers, schedules and
+    the autotuner; ``net_neurons`` / ``net_synapses`` / ``net_raster`` slice it per network."""
+
+    def __init__(self, nets=None, *, max_events_per_tick: int = 0, fire_log_capacity: int = 0,
+                 raster_capacity: int = 0, track_pre_count: bool = True, device: int = 0, specs=None):
+        """``nets``: workloads.Net objects (host arrays, snn_batch_create); or ``specs``: ei_network
+        specs generated on the device (snn_batch_create_ei)."""
+        o = Options(max_events_per_tick, fire_log_capacity, raster_capacity, int(track_pre_count), device)
+        descs, keep = [], []
+        h = ctypes.c_void_p()
+        if specs is not None:
+            for sp in specs:
+                d, a = _ei_desc(sp)
+                descs.append(d)
+                keep.append(a)
+            arr = (_EiDesc * len(descs))(*descs)
+            _check(lib().snn_batch_create_ei(arr, len(descs), ctypes.byref(o), ctypes.byref(h)))
+        else:
+            for net in nets:
+                d, a = _desc(net)
+                descs.append(d)
+                keep.append(a)
+            arr = (_Desc * len(descs))(*descs)
+            _check(lib().snn_batch_create(arr, len(descs), ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        del keep
+        self.n_nets = len(descs)
+        self.neuron_off = np.zeros(self.n_nets + 1, np.uint64)
+        self.synapse_off = np.zeros(self.n_nets + 1, np.uint64)
+        _check(lib().snn_batch_offsets(self._h, None, _p(self.neuron_off), _p(self.synapse_off)))
+        u = ctypes.c_void_p()
+        _check(lib().snn_batch_network(self._h, ctypes.byref(u)))
+        self.union = Network._borrowed(u, int(self.neuron_off[-1]), int(self.synapse_off[-1]), track_pre_count)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self.union._h = None
+            lib().snn_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def inject(self, net, neuron, tick, weight) -> None:
+        arrs = np.broadcast_arrays(np.atleast_1d(np.asarray(net, np.uint32)), np.atleast_1d(np.asarray(neuron, np.uint32)),
+                                   np.atleast_1d(np.asarray(tick, np.uint64)), np.atleast_1d(np.asarray(weight, np.float32)))
+        k, nb, tk, wt = (np.ascontiguousarray(x) for x in arrs)
+        _check(lib().snn_batch_inject(self._h, _p(k), _p(nb), _p(tk), _p(wt), k.shape[0]))
+
+    def run_until(self, inclusive_tick: int) -> None:
+        _check(lib().snn_batch_run_until(self._h, int(inclusive_tick)))
+
+    def run_commands(self, commands_per_net: Sequence[Sequence]) -> None:
+        """Replay each network's (kind, neuron, tick, weight) command list on the shared clock so that
+        every network sees its injects at the same scheduler poll tick as in a run of its own
+        (workloads.merge_commands)."""
+        from .workloads import merge_commands
+        cmds, owner = merge_commands(commands_per_net, self.neuron_off)
+        batch = []
+        i = 0
+        for k, n, t, w in cmds:
+            if k == 0:
+                batch.append((owner[i], n - int(self.neuron_off[owner[i]]), t, w))
+                i += 1
+                continue
+            if batch:
+                a = np.array(batch, dtype=np.float64)
+                self.inject(a[:, 0], a[:, 1], a[:, 2].astype(np.uint64), a[:, 3].astype(np.float32))
+                batch = []
+            self.run_until(t)
+        if batch:
+            a = np.array(batch, dtype=np.float64)
+            self.inject(a[:, 0], a[:, 1], a[:, 2].astype(np.uint64), a[:, 3].astype(np.float32))
+
+    def net_counts(self):
+        """(delivered[n_nets], fired[n_nets]); delivered counts external injects, as the reference does."""
+        d = np.zeros(self.n_nets, np.uint64) if self.union.track_pre_count else None
+        f = np.zeros(self.n_nets, np.uint64)
+        _check(lib().snn_batch_net_counts(self._h, _p(d), _p(f)))
+        return d, f
+
+    def _slices(self, k):
+        return (slice(int(self.neuron_off[k]), int(self.neuron_off[k + 1])),
+                slice(int(self.synapse_off[k]), int(self.synapse_off[k + 1])))
+
+    def net_neurons(self, k: int, neurons=None) -> Dict[str, np.ndarray]:
+        a = neurons if neurons is not None else self.union.neurons()
+        ns, _ = self._slices(k)
+        return {key: (v[:, ns] if v.ndim == 2 else v[ns]) for key, v in a.items()}
+
+    def net_synapses(self, k: int, synapses=None) -> Dict[str, np.ndarray]:
+        a = synapses if synapses is not None else self.union.synapses()
+        _, es = self._slices(k)
+        return {key: v[es] for key, v in a.items()}
+
+    def net_raster(self, k: int, raster=None) -> np.ndarray:
+        """Network k's fires in processing order, as network-local neuron indices."""
+        r = raster if raster is not None else self.union.raster()
+        lo, hi = int(self.neuron_off[k]), int(self.neuron_off[k + 1])
+        return (r[(r >= lo) & (r < hi)] - lo).astype(np.uint32)

@@ -519,25 +519,38 @@ __global__ void k_gather_u32(const uint32_t* slot_of, const uint32_t* src, uint6
 }
 
 // ------------------------------------------------------------------------------------ synthetic E/I networks
-// Device generator for the seeded E/I network family of snncuda_amd/workloads.py (ei_network with
-// one code per synapse): every draw is the same counter hash, so a generated network is identical
-// to the host-built one. Connectome order is source-major (edge j of source s has index
-// row[s] + j); slots are sorted by (source, delay, j) exactly as build_host_net sorts them.
+// Device generator for the seeded E/I network family of snncuda_amd/workloads.py (ei_network), for
+// one network or a batch of them laid out as their union: every draw is the same counter hash, so a
+// generated network is identical to the host-built one. Connectome order is source-major (edge j of
+// union source s has index srow[s] + j); slots are sorted by (source, lag of code 0, j) and virtual
+// edges by (source, lag, j, code index), exactly as build_host_net sorts them.
 
-struct GenParams {
-    uint64_t n, nall, n_exc;
+constexpr uint32_t kGenBins = 128;  // lags 1..127 (delay_max <= 63, code_max <= 63)
+
+struct GenNet {
+    uint64_t n, nall, n_exc, noff;  // network neurons, plus sources, excitatory, first union neuron
+    uint64_t key[10];               // stream keys of the network's seed (index = stream)
     float w_exc, w_inh;
     uint32_t delay_max, n_comps;
     uint32_t comps[4];
-    bool plastic;
-    uint64_t stream_key[8];
-    const uint64_t* row;
+    uint32_t ncode, code_max, plastic, pad;
+};
+
+struct GenParams {
+    const GenNet* nets;
+    uint32_t n_nets;
+    uint32_t W;  // lmax + 1
+    uint64_t nall;
+    const uint64_t* srow;  // slot (connectome) row starts, nall + 1
+    const uint64_t* vrow;  // virtual-edge row starts, nall + 1
     uint32_t* grp;
     uint32_t* tgt;
     float* w;
     uint8_t* meta;
     uint16_t* delay;
     uint32_t* slot_of;
+    uint32_t* ve;        // null when every synapse has one code
+    uint8_t* expected;   // null when uniform
     uint32_t* incnt;
     const uint64_t* in_ptr;
     uint32_t* tpos;
@@ -561,95 +574,164 @@ struct GenEdge {
     uint8_t meta;
 };
 
-__device__ __forceinline__ uint32_t gen_delay(const GenParams& P, uint64_t s, uint64_t ctr) {
-    return 1 + gbelow(ghash(P.stream_key[s < P.n ? 5 : 7], ctr), P.delay_max);
+// the network of union neuron s (nets sorted by noff)
+__device__ __forceinline__ uint32_t gen_net_of(const GenParams& P, uint64_t s) {
+    uint32_t lo = 0, hi = P.n_nets - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (P.nets[mid].noff <= s) lo = mid; else hi = mid - 1;
+    }
+    return lo;
 }
 
-__device__ __forceinline__ uint32_t gen_target(const GenParams& P, uint64_t s, uint64_t ctr) {
-    if (s < P.n) {
-        uint32_t t = gbelow(ghash(P.stream_key[1], ctr), P.n - 1);
+__device__ __forceinline__ uint32_t gen_delay(const GenNet& N, uint64_t s, uint64_t ctr) {
+    return 1 + gbelow(ghash(N.key[s < N.n ? 5 : 7], ctr), N.delay_max);
+}
+
+__device__ __forceinline__ uint32_t gen_code(const GenNet& N, uint64_t ctr, uint32_t c) {
+    return N.code_max ? gbelow(ghash(N.key[8], (ctr << 4) + c), N.code_max + 1) : 0u;
+}
+
+// network-local target
+__device__ __forceinline__ uint32_t gen_target(const GenNet& N, uint64_t s, uint64_t ctr) {
+    if (s < N.n) {
+        uint32_t t = gbelow(ghash(N.key[1], ctr), N.n - 1);
         return t + (t >= s ? 1u : 0u);
     }
-    return gbelow(ghash(P.stream_key[2], ctr), P.n);
+    return gbelow(ghash(N.key[2], ctr), N.n);
 }
 
-__device__ GenEdge gen_edge(const GenParams& P, uint64_t s, uint64_t j) {
+// edge j of network-local source s
+__device__ GenEdge gen_edge(const GenNet& N, uint64_t s, uint64_t j) {
     const uint64_t ctr = (s << 24) + j;
     GenEdge g;
-    g.tgt = gen_target(P, s, ctr);
-    g.delay = gen_delay(P, s, ctr);
+    g.tgt = (uint32_t)(N.noff + gen_target(N, s, ctr));
+    g.delay = gen_delay(N, s, ctr);
     uint32_t comp, rec;
-    if (s < P.n) {
-        const bool exc = s < P.n_exc;
-        const float u = gunif24(ghash(P.stream_key[3], ctr));
-        g.w = exc ? u * P.w_exc : u * P.w_inh;
+    if (s < N.n) {
+        const bool exc = s < N.n_exc;
+        const float u = gunif24(ghash(N.key[3], ctr));
+        g.w = exc ? u * N.w_exc : u * N.w_inh;
         rec = exc ? kAmpa : kGabaA;
-        comp = P.comps[gbelow(ghash(P.stream_key[4], ctr), P.n_comps)];
+        comp = N.comps[gbelow(ghash(N.key[4], ctr), N.n_comps)];
     } else {
-        g.w = gunif24(ghash(P.stream_key[6], ctr)) * 0.5f;
+        g.w = gunif24(ghash(N.key[6], ctr)) * 0.5f;
         rec = kAmpa;
         comp = kBasal;
     }
-    g.meta = (uint8_t)(rec | (comp << 2) | ((P.plastic ? 1u : 0u) << 4));
+    g.meta = (uint8_t)(rec | (comp << 2) | ((N.plastic ? 1u : 0u) << 4));
     return g;
 }
 
-// pass 1: per-source delay histogram -> group table; in-degree counts of plastic synapses
-__global__ void __launch_bounds__(kBlock) k_gen_pass1(GenParams P) {
-    __shared__ uint32_t hist[kBlock / kWave][64];
+__device__ __forceinline__ uint32_t gen_lag(uint32_t delay, uint32_t code) {
+    const uint32_t l = delay + code;
+    return l == 0 ? 1u : l;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// exclusive scan of a wave's kGenBins-entry LDS histogram, in place
+__device__ __forceinline__ void wave_bins_excl_scan(uint32_t* h, uint32_t lane) {
+    const uint32_t a = h[2 * lane], b = h[2 * lane + 1];
+    const uint32_t incl = wave_incl_scan(a + b);
+    wave_lds_sync();
+    h[2 * lane] = incl - a - b;
+    h[2 * lane + 1] = incl - b;
+    wave_lds_sync();
+}
+
+// Stable placement of one chunk of up to 64 items (lanes in item order) by key: item gets
+// run[key] + its rank among this chunk's lanes with the same key; run[key] advances.
+__device__ __forceinline__ uint32_t wave_place(bool ok, uint32_t key, uint32_t* run, uint32_t lane) {
+    uint64_t rest = __ballot(ok);
+    uint32_t pos = 0;
+    const uint64_t lt = (1ull << lane) - 1;
+    while (rest) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(rest);
+        const uint32_t k = __shfl(key, (int)leader, kWave);
+        const uint64_t m = __ballot(ok && key == k);
+        const uint32_t base = run[k];
+        if (ok && key == k) pos = base + (uint32_t)__popcll(m & lt);
+        wave_lds_sync();
+        if (lane == leader) run[k] = base + (uint32_t)__popcll(m);
+        wave_lds_sync();
+        rest &= ~m;
+    }
+    return pos;
+}
+
+// pass 1: per-source lag histogram of the virtual edges -> group table; in-degree counts of plastic
+// synapses
+__global__ void __launch_bounds__(kBlock) k_gen_count(GenParams P) {
+    __shared__ uint32_t hist[kBlock / kWave][kGenBins];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wv = threadIdx.x / kWave;
     const uint64_t s = (uint64_t)blockIdx.x * (kBlock / kWave) + wv;
-    const bool valid = s < P.nall;
-    hist[wv][lane] = 0;
-    __syncthreads();
-    const uint64_t r0 = valid ? P.row[s] : 0, len = valid ? P.row[s + 1] - r0 : 0;
+    if (s >= P.nall) return;  // whole wave; no block barriers below
+    uint32_t* h = hist[wv];
+    h[2 * lane] = 0;
+    h[2 * lane + 1] = 0;
+    wave_lds_sync();
+    const GenNet& N = P.nets[gen_net_of(P, s)];
+    const uint64_t sl = s - N.noff;
+    const uint64_t len = P.srow[s + 1] - P.srow[s];
     for (uint64_t j = lane; j < len; j += kWave) {
-        const uint64_t ctr = (s << 24) + j;
-        atomicAdd(&hist[wv][gen_delay(P, s, ctr)], 1u);
-        if (P.plastic) atomicAdd(&P.incnt[gen_target(P, s, ctr)], 1u);
+        const uint64_t ctr = (sl << 24) + j;
+        const uint32_t d = gen_delay(N, sl, ctr);
+        for (uint32_t c = 0; c < N.ncode; ++c) atomicAdd(&h[gen_lag(d, gen_code(N, ctr, c))], 1u);
+        if (N.plastic) atomicAdd(&P.incnt[N.noff + gen_target(N, sl, ctr)], 1u);
     }
-    __syncthreads();
-    if (valid) {
-        // lane L-1 holds the count of lag L; exclusive scan gives the group starts
-        const uint32_t c = (lane + 1 <= P.delay_max) ? hist[wv][lane + 1] : 0u;
-        const uint32_t incl = wave_incl_scan(c);
-        const uint32_t W = P.delay_max + 1;
-        if (lane < P.delay_max) P.grp[s * W + lane] = incl - c;
-        if (lane == 0) P.grp[s * W + P.delay_max] = (uint32_t)len;
-    }
+    wave_lds_sync();
+    wave_bins_excl_scan(h, lane);
+    // h[L] = virtual edges of lag < L; group L starts at h[L]
+    for (uint32_t L = 1 + lane; L < P.W; L += kWave) P.grp[s * P.W + L - 1] = h[L];
+    if (lane == 0) P.grp[s * P.W + P.W - 1] = (uint32_t)(P.vrow[s + 1] - P.vrow[s]);
 }
 
-// pass 2: stable placement of each source's synapses by delay, plus target-major STDP records
-__global__ void __launch_bounds__(kBlock) k_gen_pass2(GenParams P) {
+// pass 2: stable placement of each source's synapses by lag of code 0 and of its virtual edges by
+// lag, plus target-major STDP records
+__global__ void __launch_bounds__(kBlock) k_gen_place(GenParams P) {
+    __shared__ uint32_t runs[kBlock / kWave][kGenBins];
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint64_t s = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const uint32_t wv = threadIdx.x / kWave;
+    const uint64_t s = (uint64_t)blockIdx.x * (kBlock / kWave) + wv;
     if (s >= P.nall) return;  // whole wave; no block barriers below
-    const uint32_t W = P.delay_max + 1;
-    const uint64_t r0 = P.row[s], len = P.row[s + 1] - r0;
-    uint32_t running = lane < P.delay_max ? P.grp[s * W + lane] : 0u;  // next free position of lag lane+1
-    const uint64_t lt = (1ull << lane) - 1;
+    uint32_t* run = runs[wv];
+    const GenNet& N = P.nets[gen_net_of(P, s)];
+    const uint64_t sl = s - N.noff;
+    const uint64_t r0 = P.srow[s], len = P.srow[s + 1] - r0;
+    // slots: histogram of lag of code 0, scanned, then stable placement in connectome order
+    run[2 * lane] = 0;
+    run[2 * lane + 1] = 0;
+    wave_lds_sync();
+    for (uint64_t j = lane; j < len; j += kWave) {
+        const uint64_t ctr = (sl << 24) + j;
+        atomicAdd(&run[gen_lag(gen_delay(N, sl, ctr), gen_code(N, ctr, 0))], 1u);
+    }
+    wave_lds_sync();
+    wave_bins_excl_scan(run, lane);
     for (uint64_t c = 0; c < len; c += kWave) {
         const uint64_t j = c + lane;
         const bool ok = j < len;
         GenEdge g{};
-        if (ok) g = gen_edge(P, s, j);
-        uint32_t pos = 0;
-        for (uint32_t L = 1; L <= P.delay_max; ++L) {
-            const uint64_t m = __ballot(ok && g.delay == L);
-            if (!m) continue;
-            const uint32_t base = __shfl(running, (int)(L - 1), kWave);
-            if (ok && g.delay == L) pos = base + (uint32_t)__popcll(m & lt);
-            if (lane == L - 1) running += (uint32_t)__popcll(m);
+        uint32_t l0 = 0;
+        if (ok) {
+            g = gen_edge(N, sl, j);
+            l0 = gen_lag(g.delay, gen_code(N, (sl << 24) + j, 0));
         }
+        const uint32_t pos = wave_place(ok, l0, run, lane);
         if (ok) {
             const uint64_t slot = r0 + pos;
             P.tgt[slot] = g.tgt;
             P.w[slot] = g.w;
             P.meta[slot] = g.meta;
             P.delay[slot] = (uint16_t)g.delay;
+            if (P.expected) P.expected[slot] = (uint8_t)(N.ncode > 2 ? N.ncode : 2);
             P.slot_of[r0 + j] = (uint32_t)slot;
-            if (P.plastic) {
+            if (N.plastic) {
                 const uint64_t tp = P.in_ptr[g.tgt] + atomicAdd(&P.incnt[g.tgt], 1u);
                 P.tpos[slot] = (uint32_t)tp;
                 StdpRec r;
@@ -659,8 +741,31 @@ __global__ void __launch_bounds__(kBlock) k_gen_pass2(GenParams P) {
                 r.aux = 0;  // wmax table entry 0 = 2.0
                 P.stdp[tp] = r;
                 P.tpos_conn[tp] = (uint32_t)(r0 + j);
+            } else if (P.tpos) {
+                P.tpos[slot] = kNone;
             }
         }
+    }
+    if (!P.ve) return;
+    // virtual edges in (j, code) order, placed stably by lag; each refers to its synapse's slot
+    __threadfence();  // this wave's slot_of stores are read back below
+    wave_lds_sync();
+    for (uint32_t L = lane; L < kGenBins; L += kWave) run[L] = (L >= 1 && L < P.W) ? P.grp[s * P.W + L - 1] : 0u;
+    wave_lds_sync();
+    const uint64_t v0 = P.vrow[s], nv = P.vrow[s + 1] - v0;
+    for (uint64_t c = 0; c < nv; c += kWave) {
+        const uint64_t v = c + lane;
+        const bool ok = v < nv;
+        uint32_t L = 0;
+        uint64_t j = 0;
+        if (ok) {
+            j = v / N.ncode;
+            const uint32_t k = (uint32_t)(v - j * N.ncode);
+            const uint64_t ctr = (sl << 24) + j;
+            L = gen_lag(gen_delay(N, sl, ctr), gen_code(N, ctr, k));
+        }
+        const uint32_t pos = wave_place(ok, L, run, lane);
+        if (ok) P.ve[v0 + pos] = P.slot_of[r0 + j];
     }
 }
 
@@ -669,7 +774,6 @@ __global__ void k_widen(const uint32_t* in, uint64_t n, uint64_t* out) {
         out[i] = in[i];
 }
 
-uint64_t stream_key(uint64_t seed, uint64_t stream) { return mix64(seed * kGold + stream); }
 
 // ------------------------------------------------------------------------------------ host side
 
@@ -737,6 +841,13 @@ struct snn_net {
     bool profiling = false;
     double kms[KCount_] = {0};
     uint64_t klaunch[KCount_] = {0};
+    // launch schedule (results are identical under every schedule)
+    uint32_t ncu = 256;
+    uint32_t graph_ticks = 0;     // ticks per HIP graph replay; 0: stream launches
+    uint32_t hist_cap_parts = 0;  // hist holds hist_cap_parts * max_nwg counters
+    uint32_t max_nwg = 0;
+    uint32_t tune_trial = 0;      // > 0: the next long enough run_until autotunes with chunks of this many ticks
+    std::string tune_log;
 
     ~snn_net() {
         if (device >= 0) (void)hipSetDevice(device);
@@ -745,6 +856,17 @@ struct snn_net {
         for (auto& b : owned) (void)hipFree(b.p);
         if (stream) (void)hipStreamDestroy(stream);
     }
+};
+
+// Independent networks run as one network: their disjoint union. Every state write of a tick is
+// local to one target and its incoming synapses, and the order of the events that reach a target is
+// the order they have in that target's own network, so each network's results equal a run of that
+// network alone (tests check this against the oracle per network).
+struct snn_batch {
+    std::unique_ptr<snn_net> net;
+    std::vector<uint64_t> noff, eoff;  // n_nets + 1 neuron / synapse offsets into the union
+    std::vector<uint32_t> inj_net;     // injects through snn_batch_inject: network and tick (clamped
+    std::vector<uint64_t> inj_tick;    // to the next poll tick, as the scheduler does)
 };
 
 namespace {
@@ -801,17 +923,18 @@ void dispatch_fold(snn_net* net) {
 
 struct Timer {
     snn_net* net;
+    bool enabled = true;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     int cur = -1;
     hipEvent_t a = nullptr;
     void begin(int id) {
-        if (!net->profiling) return;
+        if (!enabled || !net->profiling) return;
         cur = id;
         check(hipEventCreate(&a), "hipEventCreate");
         check(hipEventRecord(a, net->stream), "hipEventRecord");
     }
     void end() {
-        if (!net->profiling) return;
+        if (!enabled || !net->profiling) return;
         hipEvent_t b;
         check(hipEventCreate(&b), "hipEventCreate");
         check(hipEventRecord(b, net->stream), "hipEventRecord");
@@ -889,6 +1012,149 @@ int guarded(F&& f) {
     } catch (const std::exception& e) {
         return fail(SNN_EDEVICE, e.what());
     }
+}
+
+// ------------------------------------------------------------------------------------ launch schedules
+// A schedule changes only how a tick is launched (partition size, scatter workgroups, ticks per HIP
+// graph replay); every schedule gives the same results, so it may change between any two ticks.
+
+struct Schedule {
+    uint32_t pshift, nwg, graph;
+};
+
+Schedule current_schedule(const snn_net* net) { return {net->S.pshift, net->S.nwg, net->graph_ticks}; }
+
+void apply_schedule(snn_net* net, const Schedule& sc) {
+    Step& S = net->S;
+    if (sc.pshift < 10 || sc.pshift > 12) throw std::invalid_argument("partition_shift must be in [10, 12]");
+    const uint64_t parts = ((uint64_t)S.n + (1ull << sc.pshift) - 1) >> sc.pshift;
+    if (parts > net->hist_cap_parts || parts * 4 > 65536) throw std::invalid_argument("partition_shift too small");
+    if (sc.nwg < 1 || sc.nwg > net->max_nwg) throw std::invalid_argument("scatter_workgroups out of range");
+    if (sc.graph > 1024) throw std::invalid_argument("graph_ticks above 1024");
+    S.pshift = sc.pshift;
+    S.nparts = (uint32_t)parts;
+    S.nwg = sc.nwg;
+    net->graph_ticks = sc.graph;
+}
+
+// The launches of graph_ticks consecutive ticks captured as one HIP graph. Kernel arguments are
+// fixed for the length of one run_until call, so a graph lives for one call.
+struct TickGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    Schedule sc{};
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~TickGraph() { reset(); }
+};
+
+void ensure_graph(snn_net* net, uint32_t t0, TickGraph& g) {
+    const Schedule sc = current_schedule(net);
+    if (g.exec && g.sc.pshift == sc.pshift && g.sc.nwg == sc.nwg && g.sc.graph == sc.graph) return;
+    g.reset();
+    Timer off{net};
+    off.enabled = false;
+    check(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    for (uint32_t i = 0; i < sc.graph; ++i) launch_tick(net, t0, off);
+    check(hipStreamEndCapture(net->stream, &g.graph), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    g.sc = sc;
+}
+
+// k ticks of the current run: graph replays where graph_ticks fit, stream launches for the rest.
+void run_ticks(snn_net* net, uint32_t t0, uint64_t k, Timer& tm, TickGraph& g) {
+    uint64_t i = 0;
+    const uint32_t G = net->profiling ? 0 : net->graph_ticks;
+    if (G && k >= G) {
+        ensure_graph(net, t0, g);
+        for (; i + G <= k; i += G) check(hipGraphLaunch(g.exec, net->stream), "hipGraphLaunch");
+    }
+    for (; i < k; ++i) launch_tick(net, t0, tm);
+}
+
+// Online autotuning on the run's own ticks. Coordinate search, one schedule field at a time (graph
+// length, then scatter workgroups, then partition size) around the best so far; every candidate
+// runs `trial` ticks in each of two interleaved rounds, timed with HIP events on the stream, and
+// the lowest time per tick wins. Returns the ticks consumed (0 when the run is too short, in which
+// case tuning stays pending for a later call).
+uint64_t autotune(snn_net* net, uint32_t t0, uint64_t nt, Timer& tm, TickGraph& g) {
+    const uint32_t trial = net->tune_trial;
+    auto graph_cands = [&](Schedule b) {
+        std::vector<Schedule> c{b, b};
+        c[0].graph = 0;
+        c[1].graph = std::min<uint32_t>(16, trial);
+        return c;
+    };
+    auto nwg_cands = [&](Schedule b) {
+        std::vector<Schedule> c;
+        for (uint32_t m : {net->ncu / 2, net->ncu, 2 * net->ncu, 4 * net->ncu}) {
+            if (m < 1 || m > net->max_nwg) continue;
+            b.nwg = m;
+            c.push_back(b);
+        }
+        return c;
+    };
+    auto part_cands = [&](Schedule b) {
+        std::vector<Schedule> c;
+        for (uint32_t ps = 10; ps <= 12; ++ps) {
+            const uint64_t parts = ((uint64_t)net->S.n + (1ull << ps) - 1) >> ps;
+            if (parts > net->hist_cap_parts || parts * 4 > 65536) continue;
+            if (ps > 10 && net->S.n <= (1u << (ps - 1))) continue;  // one partition already
+            b.pshift = ps;
+            c.push_back(b);
+        }
+        return c;
+    };
+    const Schedule start = current_schedule(net);
+    const uint64_t need = 2ull * trial * (graph_cands(start).size() + nwg_cands(start).size() + part_cands(start).size());
+    if (trial == 0 || nt < need + trial) return 0;
+    hipEvent_t a = nullptr, b = nullptr;
+    check(hipEventCreate(&a), "hipEventCreate");
+    check(hipEventCreate(&b), "hipEventCreate");
+    uint64_t used = 0;
+    Schedule best = start;
+    std::string log;
+    try {
+        for (int dim = 0; dim < 3; ++dim) {
+            const std::vector<Schedule> cands = dim == 0 ? graph_cands(best) : dim == 1 ? nwg_cands(best) : part_cands(best);
+            std::vector<float> t(cands.size(), 1e30f);
+            for (int round = 0; round < 2; ++round)
+                for (size_t c = 0; c < cands.size(); ++c) {
+                    apply_schedule(net, cands[c]);
+                    if (cands[c].graph && trial >= cands[c].graph) ensure_graph(net, t0, g);
+                    check(hipEventRecord(a, net->stream), "hipEventRecord");
+                    run_ticks(net, t0, trial, tm, g);
+                    check(hipEventRecord(b, net->stream), "hipEventRecord");
+                    check(hipEventSynchronize(b), "hipEventSynchronize");
+                    float ms = 0;
+                    check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+                    t[c] = std::min(t[c], ms / trial);
+                    used += trial;
+                }
+            size_t k = 0;
+            for (size_t c = 1; c < cands.size(); ++c)
+                if (t[c] < t[k]) k = c;
+            best = cands[k];
+            for (size_t c = 0; c < cands.size(); ++c)
+                log += "shift=" + std::to_string(cands[c].pshift) + " wgs=" + std::to_string(cands[c].nwg) +
+                       " graph=" + std::to_string(cands[c].graph) + " ms/tick=" + std::to_string(t[c]) +
+                       (c == k ? " *" : "") + "\n";
+        }
+    } catch (...) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        throw;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    apply_schedule(net, best);
+    net->tune_log = log;
+    net->tune_trial = 0;
+    return used;
 }
 
 std::vector<float> host_decay_table() {
@@ -1002,9 +1268,14 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     int ncu = 256;
     hipDeviceProp_t pr;
     if (hipGetDeviceProperties(&pr, o.device) == hipSuccess) ncu = pr.multiProcessorCount;
-    S.nwg = (uint32_t)std::max(1, ncu);  // one scatter workgroup per CU (measured best with 1024-neuron partitions)
+    net->ncu = (uint32_t)std::max(1, ncu);
+    S.nwg = net->ncu;  // one scatter workgroup per CU (measured best with 1024-neuron partitions)
     if (const char* e = std::getenv("SNNCUDA_SCATTER_WGS")) S.nwg = (uint32_t)std::max(1, std::atoi(e));
-    S.hist = dev_alloc<uint32_t>(own, (uint64_t)S.nparts * S.nwg);
+    // the histogram holds the largest schedule the autotuner may pick: 1024-neuron partitions and
+    // four workgroups per CU
+    net->max_nwg = std::max<uint32_t>(S.nwg, 4 * net->ncu);
+    net->hist_cap_parts = (uint32_t)(((uint64_t)n + 1023) >> 10);
+    S.hist = dev_alloc<uint32_t>(own, (uint64_t)net->hist_cap_parts * net->max_nwg);
     S.prec = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
     S.ptl = dev_alloc<uint16_t>(own, ev_cap, false);
     S.stage = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
@@ -1023,6 +1294,208 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     net->grid = (uint32_t)std::max(64, cus * 4);
     net->fold_grid = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock, 65535ull * 4);
     check(hipDeviceSynchronize(), "create");
+}
+
+NetDesc to_net_desc(const snn_net_desc& x) {
+    NetDesc d;
+    d.n_neurons = x.n_neurons;
+    d.threshold = x.threshold;
+    d.window = x.window;
+    d.sim_thr = x.sim_thr;
+    d.max_refs = x.max_refs;
+    d.n_synapses = x.n_synapses;
+    d.source = x.source;
+    d.target = x.target;
+    d.weight = x.weight;
+    d.max_weight = x.max_weight;
+    d.delay = x.delay;
+    d.compartment = x.compartment;
+    d.receptor = x.receptor;
+    d.plastic = x.plastic;
+    d.code_ptr = x.code_ptr;
+    d.codes = x.codes;
+    return d;
+}
+
+std::unique_ptr<snn_net> create_net(const NetDesc& d, const snn_options& o) {
+    if (d.n_neurons && (!d.threshold || !d.window || !d.sim_thr || !d.max_refs))
+        throw std::invalid_argument("null neuron array");
+    if (d.n_synapses && (!d.source || !d.target || !d.weight || !d.max_weight || !d.delay || !d.compartment ||
+                         !d.receptor || !d.plastic))
+        throw std::invalid_argument("null synapse array");
+    HostNet h = build_host_net(d);
+
+    std::unique_ptr<snn_net> net(new snn_net());
+    open_net(net.get(), o, h.n, h.e, h.lmax, h.any_plastic, h.ve_identity, h.code_compact);
+    auto& own = net->owned;
+    Step& S = net->S;
+    S.row = dev_upload(own, h.row);
+    S.grp = dev_upload(own, h.grp);
+    S.ve = h.ve_identity ? nullptr : dev_upload(own, h.ve);
+    S.tgt = dev_upload(own, h.tgt);
+    S.w = dev_upload(own, h.w);
+    S.meta = dev_upload(own, h.meta);
+    S.delay = dev_upload(own, h.delay);
+    S.expected = h.expected.empty() ? nullptr : dev_upload(own, h.expected);
+    S.expected_all = h.expected_all ? h.expected_all : 2;
+    if (h.any_plastic) {
+        S.tpos = dev_upload(own, h.tpos);
+        S.in_ptr = dev_upload(own, h.in_ptr);
+        S.stdp = dev_upload(own, h.stdp);
+        S.upd_count = dev_alloc<uint32_t>(own, h.stdp.size());
+        S.ltp_count = dev_alloc<uint32_t>(own, h.stdp.size());
+        S.wmax = dev_upload(own, h.wmax);
+        net->tpos_conn_dev = dev_upload(own, h.tpos_conn);
+        net->n_tpos = h.stdp.size();
+    }
+    S.np = dev_upload(own, h.params);
+    net->slot_of = dev_upload(own, h.slot_of);
+    finish_net(net.get(), o, h.nve, h.params);
+    return net;
+}
+
+uint64_t stream_key_of(uint64_t seed, uint64_t stream) { return mix64(seed * kGold + stream); }
+
+// Generates networks g[0..m) on the device as one union network (m = 1: a single network). noff /
+// eoff receive the m + 1 neuron and synapse offsets.
+std::unique_ptr<snn_net> generate_nets(const snn_ei_desc* g, uint64_t m, const snn_options& o,
+                                       std::vector<uint64_t>& noff, std::vector<uint64_t>& eoff) {
+    noff.assign(m + 1, 0);
+    eoff.assign(m + 1, 0);
+    std::vector<GenNet> nets(m);
+    uint32_t lmax = 1, exp_all = 0;
+    bool identity = true, exp_uniform = true, all_one = true, any_plastic = false;
+    for (uint64_t k = 0; k < m; ++k) {
+        const snn_ei_desc& d = g[k];
+        const std::string at = " (network " + std::to_string(k) + ")";
+        if (d.n < 2 || !d.fanout) throw std::invalid_argument("need at least two network neurons and a fan-out array" + at);
+        if (d.delay_max < 1 || d.delay_max > 63) throw std::invalid_argument("delay_max must be in [1, 63]" + at);
+        if (d.n_comps < 1 || d.n_comps > 4) throw std::invalid_argument("n_comps must be in [1, 4]" + at);
+        if (d.window > 65535 || d.max_refs < 1) throw std::invalid_argument("bad temporal parameters" + at);
+        if (d.n_codes > kCodeMax || d.code_max > 63 || (d.n_codes == 0 && d.code_max != 0))
+            throw std::invalid_argument("n_codes must be in [1, 8] and code_max in [0, 63]" + at);
+        for (uint32_t c = 0; c < d.n_comps; ++c)
+            if (d.comps[c] > 3) throw std::invalid_argument("bad compartment" + at);
+        if (d.source_fanout >= (1u << 24)) throw std::invalid_argument("fan-out must be below 2^24" + at);
+        GenNet& N = nets[k];
+        N.n = d.n;
+        N.nall = d.n + d.n_sources;
+        N.n_exc = d.n_exc;
+        N.noff = noff[k];
+        for (uint32_t st = 0; st < 10; ++st) N.key[st] = stream_key_of(d.seed, st);
+        N.w_exc = d.w_exc;
+        N.w_inh = d.w_inh;
+        N.delay_max = d.delay_max;
+        N.n_comps = d.n_comps;
+        for (int c = 0; c < 4; ++c) N.comps[c] = d.comps[c];
+        N.ncode = d.n_codes ? d.n_codes : 1;
+        N.code_max = d.code_max;
+        N.plastic = d.plastic ? 1u : 0u;
+        noff[k + 1] = noff[k] + N.nall;
+        if (noff[k + 1] >= 0xFFFFFFFFull) throw std::invalid_argument("too many neurons");
+        lmax = std::max<uint32_t>(lmax, d.delay_max + d.code_max);
+        if (N.ncode != 1 || N.code_max != 0) identity = false;
+        const uint32_t ex = N.ncode > 2 ? N.ncode : 2;
+        if (k == 0) exp_all = ex;
+        else if (ex != exp_all) exp_uniform = false;
+        if (d.max_refs != 1) all_one = false;
+        any_plastic |= N.plastic != 0;
+    }
+    const uint64_t n = noff[m];
+    const uint32_t W = lmax + 1;
+    if (W > kGenBins) throw std::invalid_argument("delay_max + code_max above 127");
+    if (n * W > (1ull << 31)) throw std::invalid_argument("neurons x (max lag + 1) exceeds the group-table limit");
+    std::vector<uint64_t> srow(n + 1, 0), vrow(n + 1, 0);
+    for (uint64_t k = 0; k < m; ++k) {
+        const snn_ei_desc& d = g[k];
+        const uint64_t b = noff[k];
+        for (uint64_t i = 0; i < nets[k].nall; ++i) {
+            const uint64_t f = i < d.n ? d.fanout[i] : d.source_fanout;
+            if (f >= (1u << 24)) throw std::invalid_argument("fan-out must be below 2^24");
+            srow[b + i + 1] = srow[b + i] + f;
+            vrow[b + i + 1] = vrow[b + i] + f * nets[k].ncode;
+        }
+        eoff[k + 1] = srow[noff[k + 1]];
+    }
+    const uint64_t e = srow[n], nve = vrow[n];
+    if (e >= 0xFFFFFFFFull) throw std::invalid_argument("more than 2^32-2 synapses on one device");
+
+    std::unique_ptr<snn_net> net(new snn_net());
+    const bool compact = exp_uniform && exp_all == 2 && all_one;
+    open_net(net.get(), o, (uint32_t)n, e, lmax, any_plastic, identity, compact);
+    auto& own = net->owned;
+    Step& S = net->S;
+    hipStream_t st = net->stream;
+    GenParams gp{};
+    gp.nets = dev_upload(own, nets);
+    gp.n_nets = (uint32_t)m;
+    gp.W = W;
+    gp.nall = n;
+    gp.srow = dev_upload(own, srow);
+    S.row = dev_upload(own, vrow);
+    gp.vrow = S.row;
+    gp.grp = dev_alloc<uint32_t>(own, n * W);
+    S.grp = gp.grp;
+    gp.ve = identity ? nullptr : dev_alloc<uint32_t>(own, nve, false);
+    S.ve = gp.ve;
+    gp.tgt = dev_alloc<uint32_t>(own, e, false);
+    gp.w = dev_alloc<float>(own, e, false);
+    gp.meta = dev_alloc<uint8_t>(own, e, false);
+    gp.delay = dev_alloc<uint16_t>(own, e, false);
+    S.tgt = gp.tgt;
+    S.w = gp.w;
+    S.meta = gp.meta;
+    S.delay = gp.delay;
+    gp.expected = exp_uniform ? nullptr : dev_alloc<uint8_t>(own, e, false);
+    S.expected = gp.expected;
+    S.expected_all = exp_uniform ? exp_all : 2;
+    net->slot_of = dev_alloc<uint32_t>(own, e, false);
+    gp.slot_of = net->slot_of;
+    const uint32_t rows_per_block = kBlock / kWave;
+    const dim3 ggrid((uint32_t)((n + rows_per_block - 1) / rows_per_block));
+    if (any_plastic) gp.incnt = dev_alloc<uint32_t>(own, n);
+    hipLaunchKernelGGL(k_gen_count, ggrid, dim3(kBlock), 0, st, gp);
+    if (any_plastic) {
+        // target-major STDP records
+        uint32_t* in32 = dev_alloc<uint32_t>(own, n + 1);
+        net->partials = dev_alloc<uint32_t>(own, kScanBlocks);
+        launch_scan(net.get(), PlainSrc{gp.incnt, nullptr, (uint32_t)n}, in32, &in32[n], 0, 0, nullptr);
+        uint64_t* in_ptr = dev_alloc<uint64_t>(own, n + 1);
+        hipLaunchKernelGGL(k_widen, dim3(1024), dim3(kBlock), 0, st, in32, n + 1, in_ptr);
+        check(hipMemsetAsync(gp.incnt, 0, n * 4, st), "memset");
+        uint32_t n_tpos = 0;
+        check(hipMemcpyAsync(&n_tpos, &in32[n], 4, hipMemcpyDeviceToHost, st), "read plastic count");
+        check(hipStreamSynchronize(st), "generate");
+        S.in_ptr = in_ptr;
+        gp.in_ptr = in_ptr;
+        gp.tpos = dev_alloc<uint32_t>(own, e, false);
+        gp.stdp = dev_alloc<StdpRec>(own, n_tpos, false);
+        gp.tpos_conn = dev_alloc<uint32_t>(own, n_tpos, false);
+        S.tpos = gp.tpos;
+        S.stdp = gp.stdp;
+        S.upd_count = dev_alloc<uint32_t>(own, n_tpos);
+        S.ltp_count = dev_alloc<uint32_t>(own, n_tpos);
+        std::vector<float> wm(1, 2.0f);  // max_weight 2.0 on every synapse: min(2, 2)
+        S.wmax = dev_upload(own, wm);
+        net->tpos_conn_dev = gp.tpos_conn;
+        net->n_tpos = n_tpos;
+    }
+    hipLaunchKernelGGL(k_gen_place, ggrid, dim3(kBlock), 0, st, gp);
+    std::vector<NeuronParams> params(n);
+    for (uint64_t k = 0; k < m; ++k) {
+        const snn_ei_desc& d = g[k];
+        for (uint64_t i = 0; i < nets[k].nall; ++i) {
+            NeuronParams& q = params[noff[k] + i];
+            q.threshold = i < d.n ? d.threshold : d.source_threshold;
+            q.window = d.window;
+            q.sim_thr = d.sim_thr;
+            q.max_refs = d.max_refs;
+        }
+    }
+    S.np = dev_upload(own, params);
+    check(hipStreamSynchronize(st), "generate");
+    finish_net(net.get(), o, nve, params);
+    return net;
 }
 
 }  // namespace
@@ -1050,57 +1523,7 @@ int snn_net_create(const snn_net_desc* desc, const snn_options* opts, snn_net** 
         if (opts) o = *opts;
         const int rc = check_device(o);
         if (rc) return rc;
-        NetDesc d;
-        d.n_neurons = desc->n_neurons;
-        d.threshold = desc->threshold;
-        d.window = desc->window;
-        d.sim_thr = desc->sim_thr;
-        d.max_refs = desc->max_refs;
-        d.n_synapses = desc->n_synapses;
-        d.source = desc->source;
-        d.target = desc->target;
-        d.weight = desc->weight;
-        d.max_weight = desc->max_weight;
-        d.delay = desc->delay;
-        d.compartment = desc->compartment;
-        d.receptor = desc->receptor;
-        d.plastic = desc->plastic;
-        d.code_ptr = desc->code_ptr;
-        d.codes = desc->codes;
-        if (d.n_neurons && (!d.threshold || !d.window || !d.sim_thr || !d.max_refs))
-            return fail(SNN_EINVAL, "null neuron array");
-        if (d.n_synapses && (!d.source || !d.target || !d.weight || !d.max_weight || !d.delay || !d.compartment ||
-                             !d.receptor || !d.plastic))
-            return fail(SNN_EINVAL, "null synapse array");
-        HostNet h = build_host_net(d);
-
-        std::unique_ptr<snn_net> net(new snn_net());
-        open_net(net.get(), o, h.n, h.e, h.lmax, h.any_plastic, h.ve_identity, h.code_compact);
-        auto& own = net->owned;
-        Step& S = net->S;
-        S.row = dev_upload(own, h.row);
-        S.grp = dev_upload(own, h.grp);
-        S.ve = h.ve_identity ? nullptr : dev_upload(own, h.ve);
-        S.tgt = dev_upload(own, h.tgt);
-        S.w = dev_upload(own, h.w);
-        S.meta = dev_upload(own, h.meta);
-        S.delay = dev_upload(own, h.delay);
-        S.expected = h.expected.empty() ? nullptr : dev_upload(own, h.expected);
-        S.expected_all = h.expected_all ? h.expected_all : 2;
-        if (h.any_plastic) {
-            S.tpos = dev_upload(own, h.tpos);
-            S.in_ptr = dev_upload(own, h.in_ptr);
-            S.stdp = dev_upload(own, h.stdp);
-            S.upd_count = dev_alloc<uint32_t>(own, h.stdp.size());
-            S.ltp_count = dev_alloc<uint32_t>(own, h.stdp.size());
-            S.wmax = dev_upload(own, h.wmax);
-            net->tpos_conn_dev = dev_upload(own, h.tpos_conn);
-            net->n_tpos = h.stdp.size();
-        }
-        S.np = dev_upload(own, h.params);
-        net->slot_of = dev_upload(own, h.slot_of);
-        finish_net(net.get(), o, h.nve, h.params);
-        *out = net.release();
+        *out = create_net(to_net_desc(*desc), o).release();
         return SNN_OK;
     });
 }
@@ -1113,109 +1536,24 @@ int snn_net_create_ei(const snn_ei_desc* g, const snn_options* opts, snn_net** o
         if (opts) o = *opts;
         const int rc = check_device(o);
         if (rc) return rc;
-        if (g->n < 2 || !g->fanout) return fail(SNN_EINVAL, "need at least two network neurons and a fan-out array");
-        if (g->delay_max < 1 || g->delay_max > 63) return fail(SNN_EINVAL, "delay_max must be in [1, 63]");
-        if (g->n_comps < 1 || g->n_comps > 4) return fail(SNN_EINVAL, "n_comps must be in [1, 4]");
-        if (g->window > 65535 || g->max_refs < 1) return fail(SNN_EINVAL, "bad temporal parameters");
-        const uint64_t nall = g->n + g->n_sources;
-        if (nall >= 0xFFFFFFFFull) return fail(SNN_EINVAL, "too many neurons");
-        std::vector<uint64_t> row(nall + 1, 0);
-        for (uint64_t i = 0; i < g->n; ++i) {
-            if (g->fanout[i] >= (1u << 24)) return fail(SNN_EINVAL, "fan-out must be below 2^24");
-            row[i + 1] = row[i] + g->fanout[i];
-        }
-        if (g->source_fanout >= (1u << 24)) return fail(SNN_EINVAL, "fan-out must be below 2^24");
-        for (uint64_t i = g->n; i < nall; ++i) row[i + 1] = row[i] + g->source_fanout;
-        const uint64_t e = row[nall];
-        if (e >= 0xFFFFFFFFull) return fail(SNN_EINVAL, "more than 2^32-2 synapses on one device");
-        const uint32_t lmax = g->delay_max;
+        std::vector<uint64_t> noff, eoff;
+        *out = generate_nets(g, 1, o, noff, eoff).release();
+        return SNN_OK;
+    });
+}
 
-        std::unique_ptr<snn_net> net(new snn_net());
-        const bool plastic = g->plastic != 0;
-        const bool compact = g->max_refs == 1;
-        open_net(net.get(), o, (uint32_t)nall, e, lmax, plastic, true, compact);
-        auto& own = net->owned;
-        Step& S = net->S;
-        hipStream_t st = net->stream;
-        S.row = dev_upload(own, row);
-        uint32_t* grp = dev_alloc<uint32_t>(own, nall * (lmax + 1));
-        S.grp = grp;
-        S.ve = nullptr;
-        uint32_t* tgt = dev_alloc<uint32_t>(own, e, false);
-        float* w = dev_alloc<float>(own, e, false);
-        uint8_t* meta = dev_alloc<uint8_t>(own, e, false);
-        uint16_t* delay = dev_alloc<uint16_t>(own, e, false);
-        S.tgt = tgt;
-        S.w = w;
-        S.meta = meta;
-        S.delay = delay;
-        S.expected = nullptr;
-        S.expected_all = 2;
-        net->slot_of = dev_alloc<uint32_t>(own, e, false);
-
-        GenParams gp{};
-        gp.n = g->n;
-        gp.nall = nall;
-        gp.n_exc = g->n_exc;
-        gp.w_exc = g->w_exc;
-        gp.w_inh = g->w_inh;
-        gp.delay_max = g->delay_max;
-        gp.n_comps = g->n_comps;
-        for (int k = 0; k < 4; ++k) gp.comps[k] = g->comps[k];
-        gp.plastic = plastic;
-        for (uint32_t k = 0; k < 8; ++k) gp.stream_key[k] = stream_key(g->seed, k);
-        gp.row = S.row;
-        gp.grp = grp;
-        gp.tgt = tgt;
-        gp.w = w;
-        gp.meta = meta;
-        gp.delay = delay;
-        gp.slot_of = net->slot_of;
-        uint32_t* incnt = nullptr;
-        if (plastic) {
-            incnt = dev_alloc<uint32_t>(own, nall);
-            gp.incnt = incnt;
-        }
-        const uint32_t rows_per_block = kBlock / kWave;
-        const dim3 ggrid((uint32_t)((nall + rows_per_block - 1) / rows_per_block));
-        hipLaunchKernelGGL(k_gen_pass1, ggrid, dim3(kBlock), 0, st, gp);
-        if (plastic) {
-            // target-major STDP records
-            uint32_t* in32 = dev_alloc<uint32_t>(own, nall + 1);
-            net->partials = dev_alloc<uint32_t>(own, kScanBlocks);
-            launch_scan(net.get(), PlainSrc{incnt, nullptr, (uint32_t)nall}, in32, &in32[nall], 0, 0, nullptr);
-            uint64_t* in_ptr = dev_alloc<uint64_t>(own, nall + 1);
-            hipLaunchKernelGGL(k_widen, dim3(1024), dim3(kBlock), 0, st, in32, nall + 1, in_ptr);
-            check(hipMemsetAsync(incnt, 0, nall * 4, st), "memset");
-            S.in_ptr = in_ptr;
-            uint32_t* tpos = dev_alloc<uint32_t>(own, e, false);
-            StdpRec* stdp = dev_alloc<StdpRec>(own, e, false);
-            uint32_t* tconn = dev_alloc<uint32_t>(own, e, false);
-            S.tpos = tpos;
-            S.stdp = stdp;
-            S.upd_count = dev_alloc<uint32_t>(own, e);
-            S.ltp_count = dev_alloc<uint32_t>(own, e);
-            std::vector<float> wm(1, 2.0f);  // max_weight 2.0 on every synapse: min(2, 2)
-            S.wmax = dev_upload(own, wm);
-            net->tpos_conn_dev = tconn;
-            net->n_tpos = e;
-            gp.in_ptr = in_ptr;
-            gp.tpos = tpos;
-            gp.stdp = stdp;
-            gp.tpos_conn = tconn;
-        }
-        hipLaunchKernelGGL(k_gen_pass2, ggrid, dim3(kBlock), 0, st, gp);
-        std::vector<NeuronParams> params(nall);
-        for (uint64_t i = 0; i < nall; ++i) {
-            params[i].threshold = i < g->n ? g->threshold : g->source_threshold;
-            params[i].window = g->window;
-            params[i].sim_thr = g->sim_thr;
-            params[i].max_refs = g->max_refs;
-        }
-        S.np = dev_upload(own, params);
-        check(hipStreamSynchronize(st), "generate");
-        finish_net(net.get(), o, e, params);
-        *out = net.release();
+int snn_batch_create_ei(const snn_ei_desc* nets, uint64_t n_nets, const snn_options* opts, snn_batch** out) {
+    if (!nets || !out) return fail(SNN_EINVAL, "null argument");
+    *out = nullptr;
+    if (n_nets == 0) return fail(SNN_EINVAL, "empty batch");
+    return guarded([&]() -> int {
+        snn_options o{};
+        if (opts) o = *opts;
+        const int rc = check_device(o);
+        if (rc) return rc;
+        std::unique_ptr<snn_batch> b(new snn_batch());
+        b->net = generate_nets(nets, n_nets, o, b->noff, b->eoff);
+        *out = b.release();
         return SNN_OK;
     });
 }
@@ -1223,6 +1561,202 @@ int snn_net_create_ei(const snn_ei_desc* g, const snn_options* opts, snn_net** o
 int snn_net_destroy(snn_net* net) {
     delete net;
     return SNN_OK;
+}
+
+int snn_net_get_schedule(snn_net* net, snn_schedule* out) {
+    if (!net || !out) return fail(SNN_EINVAL, "null argument");
+    out->partition_shift = net->S.pshift;
+    out->scatter_workgroups = net->S.nwg;
+    out->graph_ticks = net->graph_ticks;
+    out->reserved = 0;
+    return SNN_OK;
+}
+
+int snn_net_set_schedule(snn_net* net, const snn_schedule* sc) {
+    if (!net || !sc) return fail(SNN_EINVAL, "null argument");
+    return guarded([&]() -> int {
+        apply_schedule(net, Schedule{sc->partition_shift, sc->scatter_workgroups, sc->graph_ticks});
+        return SNN_OK;
+    });
+}
+
+int snn_net_autotune(snn_net* net, uint32_t trial_ticks) {
+    if (!net) return fail(SNN_EINVAL, "null handle");
+    if (trial_ticks > 4096) return fail(SNN_EINVAL, "trial_ticks above 4096");
+    net->tune_trial = trial_ticks;
+    return SNN_OK;
+}
+
+int snn_net_tune_log(snn_net* net, char* buf, uint64_t cap) {
+    if (!net) return fail(SNN_EINVAL, "null handle");
+    if (buf && cap) {
+        const size_t m = std::min<size_t>(cap - 1, net->tune_log.size());
+        std::memcpy(buf, net->tune_log.data(), m);
+        buf[m] = 0;
+    }
+    return (int)std::min<size_t>(net->tune_log.size(), 0x7FFFFFFF);
+}
+
+int snn_batch_create(const snn_net_desc* nets, uint64_t n_nets, const snn_options* opts, snn_batch** out) {
+    if (!out || (n_nets && !nets)) return fail(SNN_EINVAL, "null argument");
+    *out = nullptr;
+    if (n_nets == 0) return fail(SNN_EINVAL, "empty batch");
+    return guarded([&]() -> int {
+        snn_options o{};
+        if (opts) o = *opts;
+        const int rc = check_device(o);
+        if (rc) return rc;
+        std::unique_ptr<snn_batch> b(new snn_batch());
+        b->noff.assign(n_nets + 1, 0);
+        b->eoff.assign(n_nets + 1, 0);
+        uint64_t ncodes = 0;
+        bool any_codes = false;
+        for (uint64_t k = 0; k < n_nets; ++k) {
+            const snn_net_desc& x = nets[k];
+            if (x.n_neurons == 0) return fail(SNN_EINVAL, "network " + std::to_string(k) + " has no neurons");
+            if (!x.threshold || !x.window || !x.sim_thr || !x.max_refs)
+                return fail(SNN_EINVAL, "null neuron array in network " + std::to_string(k));
+            if (x.n_synapses && (!x.source || !x.target || !x.weight || !x.max_weight || !x.delay || !x.compartment ||
+                                 !x.receptor || !x.plastic))
+                return fail(SNN_EINVAL, "null synapse array in network " + std::to_string(k));
+            for (uint64_t j = 0; j < x.n_synapses; ++j)
+                if (x.source[j] >= x.n_neurons || x.target[j] >= x.n_neurons)
+                    return fail(SNN_EINVAL, "synapse endpoint out of range in network " + std::to_string(k));
+            b->noff[k + 1] = b->noff[k] + x.n_neurons;
+            b->eoff[k + 1] = b->eoff[k] + x.n_synapses;
+            if (x.code_ptr) {
+                any_codes = true;
+                ncodes += x.code_ptr[x.n_synapses] - x.code_ptr[0];
+            }
+        }
+        const uint64_t n = b->noff[n_nets], e = b->eoff[n_nets];
+        if (n >= 0xFFFFFFFFull) return fail(SNN_EINVAL, "more than 2^32-2 neurons in the batch");
+        std::vector<float> thr(n), sim(n), w(e), mw(e);
+        std::vector<uint64_t> win(n), mref(n), cptr(any_codes ? e + 1 : 0);
+        std::vector<uint32_t> src(e), tgt(e), dl(e), codes;
+        std::vector<uint8_t> comp(e), rec(e), pl(e);
+        codes.reserve(ncodes);
+        if (any_codes) cptr[0] = 0;
+        for (uint64_t k = 0; k < n_nets; ++k) {
+            const snn_net_desc& x = nets[k];
+            const uint64_t n0 = b->noff[k], e0 = b->eoff[k];
+            std::copy(x.threshold, x.threshold + x.n_neurons, thr.begin() + n0);
+            std::copy(x.window, x.window + x.n_neurons, win.begin() + n0);
+            std::copy(x.sim_thr, x.sim_thr + x.n_neurons, sim.begin() + n0);
+            std::copy(x.max_refs, x.max_refs + x.n_neurons, mref.begin() + n0);
+            for (uint64_t j = 0; j < x.n_synapses; ++j) {
+                src[e0 + j] = (uint32_t)(x.source[j] + n0);
+                tgt[e0 + j] = (uint32_t)(x.target[j] + n0);
+            }
+            std::copy(x.weight, x.weight + x.n_synapses, w.begin() + e0);
+            std::copy(x.max_weight, x.max_weight + x.n_synapses, mw.begin() + e0);
+            std::copy(x.delay, x.delay + x.n_synapses, dl.begin() + e0);
+            std::copy(x.compartment, x.compartment + x.n_synapses, comp.begin() + e0);
+            std::copy(x.receptor, x.receptor + x.n_synapses, rec.begin() + e0);
+            std::copy(x.plastic, x.plastic + x.n_synapses, pl.begin() + e0);
+            if (any_codes)
+                for (uint64_t j = 0; j < x.n_synapses; ++j) {
+                    if (x.code_ptr)  // an empty list stands for {0}, as a missing one does
+                        codes.insert(codes.end(), x.codes + x.code_ptr[j], x.codes + x.code_ptr[j + 1]);
+                    cptr[e0 + j + 1] = codes.size();
+                }
+        }
+        NetDesc d;
+        d.n_neurons = n;
+        d.threshold = thr.data();
+        d.window = win.data();
+        d.sim_thr = sim.data();
+        d.max_refs = mref.data();
+        d.n_synapses = e;
+        d.source = src.data();
+        d.target = tgt.data();
+        d.weight = w.data();
+        d.max_weight = mw.data();
+        d.delay = dl.data();
+        d.compartment = comp.data();
+        d.receptor = rec.data();
+        d.plastic = pl.data();
+        d.code_ptr = any_codes ? cptr.data() : nullptr;
+        d.codes = any_codes ? codes.data() : nullptr;
+        b->net = create_net(d, o);
+        *out = b.release();
+        return SNN_OK;
+    });
+}
+
+int snn_batch_destroy(snn_batch* b) {
+    delete b;
+    return SNN_OK;
+}
+
+int snn_batch_network(snn_batch* b, snn_net** net) {
+    if (!b || !net) return fail(SNN_EINVAL, "null argument");
+    *net = b->net.get();
+    return SNN_OK;
+}
+
+int snn_batch_offsets(snn_batch* b, uint64_t* n_nets, uint64_t* neuron_off, uint64_t* synapse_off) {
+    if (!b) return fail(SNN_EINVAL, "null handle");
+    if (n_nets) *n_nets = b->noff.size() - 1;
+    if (neuron_off) std::copy(b->noff.begin(), b->noff.end(), neuron_off);
+    if (synapse_off) std::copy(b->eoff.begin(), b->eoff.end(), synapse_off);
+    return SNN_OK;
+}
+
+int snn_batch_inject(snn_batch* b, const uint32_t* net, const uint32_t* neuron, const uint64_t* tick,
+                     const float* weight, uint64_t n) {
+    if (!b || (n && (!net || !neuron || !tick || !weight))) return fail(SNN_EINVAL, "null argument");
+    const uint64_t nn = b->noff.size() - 1;
+    std::vector<uint32_t> g(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (net[i] >= nn) return fail(SNN_EINVAL, "inject network out of range");
+        if (neuron[i] >= b->noff[net[i] + 1] - b->noff[net[i]]) return fail(SNN_EINVAL, "inject neuron out of range");
+        g[i] = (uint32_t)(b->noff[net[i]] + neuron[i]);
+    }
+    const int rc = snn_net_inject(b->net.get(), g.data(), tick, weight, n);
+    if (rc) return rc;
+    for (uint64_t i = 0; i < n; ++i) {
+        b->inj_net.push_back(net[i]);
+        b->inj_tick.push_back(std::max<uint64_t>(tick[i], b->net->next_tick));
+    }
+    return SNN_OK;
+}
+
+int snn_batch_run_until(snn_batch* b, uint64_t inclusive_tick) {
+    if (!b) return fail(SNN_EINVAL, "null handle");
+    return snn_net_run_until(b->net.get(), inclusive_tick);
+}
+
+int snn_batch_net_counts(snn_batch* b, uint64_t* delivered, uint64_t* fired) {
+    if (!b) return fail(SNN_EINVAL, "null handle");
+    snn_net* net = b->net.get();
+    if (delivered && !net->S.pre_count) return fail(SNN_EINVAL, "per-network delivered counts need track_pre_count");
+    return guarded([&]() -> int {
+        const uint64_t nn = b->noff.size() - 1;
+        if (fired) {
+            std::vector<uint64_t> sc(net->n);
+            const int rc = snn_net_read_neurons(net, nullptr, nullptr, nullptr, sc.data(), nullptr, nullptr, nullptr);
+            if (rc) return rc;
+            for (uint64_t k = 0; k < nn; ++k) {
+                uint64_t f = 0;
+                for (uint64_t i = b->noff[k]; i < b->noff[k + 1]; ++i) f += sc[i];
+                fired[k] = f;
+            }
+        }
+        if (delivered) {
+            std::vector<uint64_t> pc(net->e);
+            const int rc = snn_net_read_synapses(net, nullptr, pc.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
+            if (rc) return rc;
+            for (uint64_t k = 0; k < nn; ++k) {
+                uint64_t d = 0;
+                for (uint64_t j = b->eoff[k]; j < b->eoff[k + 1]; ++j) d += pc[j];
+                delivered[k] = d;
+            }
+            for (size_t i = 0; i < b->inj_net.size(); ++i)
+                if (b->inj_tick[i] < net->next_tick) delivered[b->inj_net[i]] += 1;
+        }
+        return SNN_OK;
+    });
 }
 
 int snn_net_inject(snn_net* net, const uint32_t* neuron, const uint64_t* tick, const float* weight, uint64_t n) {
@@ -1280,12 +1814,18 @@ int snn_net_run_until(snn_net* net, uint64_t inclusive_tick) {
             check(hipMemcpy(S.ctl, &tick32, 4, hipMemcpyHostToDevice), "set tick");
         }
         Timer tm{net};
-        for (uint64_t t = 0; t < nt; ++t) {
-            launch_tick(net, (uint32_t)t0, tm);
-            if (net->profiling && tm.marks.size() > 4096) {
-                check(hipStreamSynchronize(net->stream), "sync");
-                tm.collect();
+        if (net->profiling) {
+            for (uint64_t t = 0; t < nt; ++t) {
+                launch_tick(net, (uint32_t)t0, tm);
+                if (tm.marks.size() > 4096) {
+                    check(hipStreamSynchronize(net->stream), "sync");
+                    tm.collect();
+                }
             }
+        } else {
+            TickGraph g;
+            const uint64_t done = net->tune_trial ? autotune(net, (uint32_t)t0, nt, tm, g) : 0;
+            run_ticks(net, (uint32_t)t0, nt - done, tm, g);
         }
         check(hipGetLastError(), "launch");
         check(hipStreamSynchronize(net->stream), "run");

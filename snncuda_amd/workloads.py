@@ -21,7 +21,7 @@ Pinned free parameters (the workload descriptions leave them open):
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import List, Optional, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -173,7 +173,8 @@ def ei_network(n: int, fanout, seed: int, *, n_sources: int, source_fanout: int,
     fan = np.broadcast_to(np.asarray(fanout, dtype=np.int64), (n,)).copy()
     spec = dict(n=n, n_sources=n_sources, fanout=fan, source_fanout=source_fanout, seed=seed, n_exc=n_exc,
                 w_exc=w_exc, w_inh=w_inh, delay_max=delay_max, compartments=tuple(compartments), plastic=plastic,
-                threshold=threshold, window=window, sim_thr=sim_thr, max_refs=max_refs)
+                threshold=threshold, window=window, sim_thr=sim_thr, max_refs=max_refs,
+                n_codes=max(1, n_codes), code_max=code_max)
     src, tgt, ctr = _fanout_edges(seed, 1, np.arange(n), fan, n, 0, True)
     ssrc, stgt, sctr = _fanout_edges(seed, 2, np.arange(n, n + n_sources), np.full(n_sources, source_fanout), n, 0, False)
     e_net = src.shape[0]
@@ -210,7 +211,7 @@ def ei_network(n: int, fanout, seed: int, *, n_sources: int, source_fanout: int,
         np.concatenate([receptor, np.full(ssrc.shape[0], AMPA, np.uint8)]),
         np.full(e, 1 if plastic else 0, np.uint8), code_ptr=code_ptr, codes=codes, name=name,
         info=dict(n=n, n_sources=n_sources, e_net=int(e_net), seed=seed, ticks=ticks,
-                  threshold=threshold, p_source=p_source, spec=spec if n_codes <= 1 and code_max == 0 else None),
+                  threshold=threshold, p_source=p_source, spec=spec if n_codes >= 1 or code_max == 0 else None),
     )
     net.commands = poisson_commands(seed, n, n_sources, p_source, ticks)
     return net
@@ -361,3 +362,118 @@ def config2_spec(seed: int = 0, n: int = 1_000_000, n_sources: int = 10_000, sou
         hit = np.nonzero(unif24(hash3(seed, 40, np.arange(n, dtype=np.uint64))) < np.float32(kick))[0]
         cmds = [(0, int(i), 0, float(threshold)) for i in hit] + cmds
     return spec, cmds
+
+
+# --------------------------------------------------------------------------- batches of networks
+
+def config3_shape(seed: int, n_range=(500, 50_000), fan_range=(50, 500)) -> Tuple[int, int]:
+    """Neuron count (log-uniform over n_range) and fan-out (uniform integer over fan_range) of the
+    batched workload's network ``seed``."""
+    u = float(unif24(hash3(seed, 50, np.zeros(1, np.uint64)))[0])
+    lo, hi = np.log(n_range[0]), np.log(n_range[1])
+    n = int(np.floor(np.exp(lo + (hi - lo) * u)))
+    f = int(fan_range[0] + below(hash3(seed, 51, np.zeros(1, np.uint64)), fan_range[1] - fan_range[0] + 1)[0])
+    return max(n_range[0], min(n_range[1], n)), f
+
+
+def config3_net(seed: int, ticks: int = 2000, threshold: float = 3.0, n_range=(500, 50_000), fan_range=(50, 500),
+                plastic: bool = True, w_inh: float = 1.0) -> Net:
+    """Network ``seed`` of the batched workload: log-uniform size, uniform fan-out, 80/20 E/I,
+    delays U{1..20}, Basal/Apical compartments, 2 code offsets per synapse from U{0..7}, 100 Poisson
+    sources at p=0.01 with fan-out 50, STDP on."""
+    n, f = config3_shape(seed, n_range, fan_range)
+    return ei_network(n, f, seed, n_sources=100, source_fanout=50, p_source=0.01, ticks=ticks,
+                      threshold=threshold, delay_max=20, compartments=(BASAL, APICAL), n_codes=2, code_max=7,
+                      plastic=plastic, w_inh=w_inh, name=f"config3_{seed}")
+
+
+def config3_spec(seed: int, ticks: int = 2000, threshold: float = 3.0, n_range=(500, 50_000), fan_range=(50, 500),
+                 plastic: bool = True, w_inh: float = 1.0):
+    """config3_net(seed, ...) as a device-generator spec plus its commands, without host arrays."""
+    n, f = config3_shape(seed, n_range, fan_range)
+    spec = dict(n=n, n_sources=100, fanout=np.full(n, f, np.int64), source_fanout=50, seed=seed,
+                n_exc=int(round(n * 0.8)), w_exc=0.5, w_inh=w_inh, delay_max=20, compartments=(BASAL, APICAL),
+                plastic=plastic, threshold=threshold, window=32, sim_thr=0.9, max_refs=1, n_codes=2, code_max=7)
+    return spec, poisson_commands(seed, n, 100, 0.01, ticks)
+
+
+def merge_commands(commands_per_net: Sequence[Sequence], neuron_off: Sequence[int]):
+    """One command list for the union of networks (network k's neuron i is neuron_off[k] + i) that
+    gives every network its injects at the same scheduler poll tick as its own command list: each
+    network's injects are issued when the shared clock reaches that network's previous run_until,
+    and the clock runs to the smallest pending run_until of any network. Returns (command list,
+    network of each inject in order)."""
+    cmds = [list(c) for c in commands_per_net]
+    pos = [0] * len(cmds)
+    out, owner = [], []
+    while True:
+        for k, c in enumerate(cmds):
+            while pos[k] < len(c) and c[pos[k]][0] == 0:
+                _, n, t, w = c[pos[k]]
+                out.append((0, int(neuron_off[k]) + int(n), int(t), float(w)))
+                owner.append(k)
+                pos[k] += 1
+        pending = [c[pos[k]][2] for k, c in enumerate(cmds) if pos[k] < len(c)]
+        if not pending:
+            return out, owner
+        m = min(pending)
+        out.append((1, 0, int(m), 0.0))
+        for k, c in enumerate(cmds):
+            if pos[k] < len(c) and c[pos[k]][0] == 1 and c[pos[k]][2] == m:
+                pos[k] += 1
+
+
+def union(nets: Sequence[Net], name: str = "union") -> Net:
+    """The disjoint union of independent networks as one network, with merged commands; info holds
+    neuron_off / synapse_off."""
+    noff = np.zeros(len(nets) + 1, np.int64)
+    eoff = np.zeros(len(nets) + 1, np.int64)
+    for k, n in enumerate(nets):
+        noff[k + 1] = noff[k] + n.n_neurons
+        eoff[k + 1] = eoff[k] + n.n_synapses
+    cat = lambda f: np.concatenate([getattr(n, f) for n in nets])
+    code_ptr = [np.zeros(1, np.uint64)]
+    base = np.uint64(0)
+    for n in nets:
+        code_ptr.append(n.code_ptr[1:] - n.code_ptr[0] + base)
+        base = code_ptr[-1][-1] if n.n_synapses else base
+    cmds, _ = merge_commands([n.commands for n in nets], noff)
+    return Net(threshold=cat("threshold"), window=cat("window"), sim_thr=cat("sim_thr"), max_refs=cat("max_refs"),
+               src=np.concatenate([n.src.astype(np.int64) + noff[k] for k, n in enumerate(nets)]).astype(np.uint32),
+               tgt=np.concatenate([n.tgt.astype(np.int64) + noff[k] for k, n in enumerate(nets)]).astype(np.uint32),
+               weight=cat("weight"), max_weight=cat("max_weight"), delay=cat("delay"), comp=cat("comp"),
+               receptor=cat("receptor"), plastic=cat("plastic"), code_ptr=np.concatenate(code_ptr).astype(np.uint64),
+               codes=cat("codes"), commands=cmds, name=name,
+               info=dict(neuron_off=noff, synapse_off=eoff))
+
+
+# --------------------------------------------------------------------------- adjoint workload
+
+def config4(seed: int = 0, n: int = 10_000, fan_in: int = 500, ticks: int = 500, threshold: float = 1.0,
+            p_in: float = 0.05, w_in: float = 0.6, delay_max: int = 20) -> Net:
+    """Network of the adjoint workload: n neurons with fan-in ``fan_in`` from uniform presynaptic
+    neurons (no self-loops), weight magnitudes U[0, 0.5); 20% of the synapses are inhibitory (GABA_A
+    or GABA_B, whose negative receptor sign negates the current, as in the other workloads), the rest
+    AMPA or NMDA; Basal/Apical compartments, delays U{1..delay_max}, codes {0}, STDP off. Drive: every
+    neuron receives an external inject of weight ``w_in`` with probability ``p_in`` per tick (soma,
+    AMPA). Connectome order is target-major."""
+    e = n * fan_in
+    tgt = np.repeat(np.arange(n, dtype=np.int64), fan_in)
+    ctr = np.arange(e, dtype=np.uint64)
+    s_ = below(hash3(seed, 60, ctr), n - 1).astype(np.int64)
+    src = s_ + (s_ >= tgt)
+    w = (unif24(hash3(seed, 61, ctr)) * np.float32(0.5)).astype(np.float32)
+    inh = unif24(hash3(seed, 62, ctr)) < np.float32(0.2)
+    receptor = (below(hash3(seed, 63, ctr), 2) + np.where(inh, 2, 0)).astype(np.uint8)
+    comp = np.where(below(hash3(seed, 64, ctr), 2) == 0, BASAL, APICAL).astype(np.uint8)
+    delay = (1 + below(hash3(seed, 65, ctr), delay_max)).astype(np.uint32)
+    thr, win, sim, mref = neurons(n, threshold, 32, 0.9, 1)
+    net = make_net((thr, win, sim, mref), src.astype(np.uint32), tgt.astype(np.uint32), w, np.full(e, 2.0, np.float32),
+                   delay, comp, receptor, np.zeros(e, np.uint8), name="config4",
+                   info=dict(n=n, fan_in=fan_in, seed=seed, ticks=ticks, threshold=threshold, p_in=p_in, w_in=w_in))
+    i = np.arange(n, dtype=np.uint64)
+    t = np.arange(ticks, dtype=np.uint64)
+    fire = unif24(hash3(seed, 66, (i[None, :] << np.uint64(32)) + t[:, None])) < np.float32(p_in)
+    tt, nn = np.nonzero(fire)
+    net.commands = [(0, int(a), int(b), float(w_in)) for b, a in zip(tt, nn)] + [(1, 0, ticks - 1, 0.0)]
+    return net

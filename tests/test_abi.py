@@ -44,3 +44,12 @@ def test_create_without_device_or_with_bad_input_fails_cleanly():
     bad.tgt[0] = bad.n_neurons + 5
     with pytest.raises(sa.SNNError):
         sa.Network(bad)
+
+
+def test_batch_create_without_device_fails_cleanly():
+    from snncuda_amd import workloads as W
+    if sa.device_count() != 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(sa.SNNError) as e:
+        sa.Batch([W.feature_net(0), W.feature_net(1)])
+    assert e.value.code == sa.SNN_ENODEVICE
