@@ -54,9 +54,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-neurons", type=int, default=20_000)
     ap.add_argument("--cpu-ticks", type=int, default=200)
-    ap.add_argument("--form", choices=("plain", "batched"), default="plain",
+    ap.add_argument("--form", choices=("plain", "batched", "adjoint"), default="plain",
                     help="plain: the 1M-neuron step (default); batched: 512 uneven networks in one launch "
-                         "sequence vs looping the plain step over the same networks")
+                         "sequence vs looping the plain step over the same networks; adjoint: forward + "
+                         "backward of the fixed-spike adjoint on the 10,000-neuron workload")
     ap.add_argument("--nets", type=int, default=512, help="batched form: networks")
     ap.add_argument("--batch-ticks", type=int, default=2000, help="batched form: ticks per network")
     ap.add_argument("--loop-nets", type=int, default=0,
@@ -177,6 +178,93 @@ def batched(args):
     }
 
 
+def adjoint_form(args):
+    """Fixed-spike adjoint on the 10,000-neuron workload (fan-in 500, 5e6 synapses, 500 ticks):
+    one step = forward (trace + checkpoints) + backward (segment recompute + adjoint recursion) of
+    the MSE loss against a perturbed-weight target. value = steps/s; the FP64 CPU adjoint of the
+    oracle (one thread) is timed on the same event lists as the CPU baseline."""
+    import numpy as np
+    import torch
+    import snncuda_amd as sa
+    from snncuda_amd import workloads as W
+    from snncuda_amd.adjoint import Adjoint, DEFAULT_PARAMS
+
+    net = W.config4(0)
+    T = 500
+    run = sa.Network(net, max_events_per_tick=1 << 24, fire_log_capacity=1 << 22, raster_capacity=1 << 24,
+                     track_pre_count=False)
+    run.record_fire_events()
+    run.run_commands(net.commands)
+    t0 = time.perf_counter()
+    adj = Adjoint(run, net, T)
+    t_build = time.perf_counter() - t0
+    dev = torch.device("cuda")
+    w = torch.tensor(net.weight, dtype=torch.float32, device=dev)
+    params = np.asarray(DEFAULT_PARAMS, np.float32)
+    target = torch.empty((T, adj.n), dtype=torch.float32, device=dev)
+    adj.forward(w + 0.05 * torch.randn(w.shape, device=dev, generator=torch.Generator(device=dev).manual_seed(0)),
+                params, target)
+    trace = torch.empty_like(target)
+    gw = torch.empty_like(w)
+
+    def step():
+        adj.forward(w, params, trace)
+        g = (2.0 / trace.numel()) * (trace - target)
+        return adj.backward(g, gw)
+
+    for _ in range(args.warmup if args.warmup < 50 else 3):
+        step()
+    K = min(args.steps, 50)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / K
+    ev = adj.n_events
+    # bytes per step: events read three times (forward, recompute, reverse) at 8 B, synapse weights
+    # gathered twice and gradients written once (4 B each per event), trace written once and read once
+    # with dL/dtrace (3 x 4 B per neuron-tick), checkpoints and segment scratch (written and read once:
+    # 2 x 20 B per neuron-tick)
+    bytes_step = ev * (3 * 8 + 3 * 4) + T * adj.n * (3 * 4 + 2 * 20)
+    line = {
+        "metric": "adjoint steps/s (10k-neuron fixed-spike adjoint, forward + backward)",
+        "value": 1.0 / dt,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": 3,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": "10,000 neurons, fan-in 500 (5e6 synapses), 500 ticks, 2 compartments, checkpoints "
+                               "every 64 ticks, MSE against a target from weights + N(0, 0.05)",
+                   "events": ev, "fires": int(run.fired), "parallelism": "1 GPU"},
+        "events_per_s": 3 * ev / dt,
+        "roofline": {"bound": "hbm", "achieved": bytes_step / dt / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bytes_step / dt / 1e9 / HBM_PEAK_GBS, "traffic": None, "bytes_per_step": bytes_step},
+        "build_seconds": t_build,
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        o = oracle.Oracle(net, log_events=True)
+        o.run_commands(net.commands)
+        ref = oracle.AdjointOracle(net, T, o.event_log())
+        tg = target.cpu().numpy().astype(np.float64)
+        wd = net.weight.astype(np.float64)
+        pd = np.asarray(DEFAULT_PARAMS, np.float64)
+        t0 = time.perf_counter()
+        ref.grad(wd, pd[:4], pd[4:], tg)
+        cdt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": 1.0 / cdt, "unit": "steps/s", "cores": 1, "kind": "port",
+                                "sample": f"oracle/adjoint_oracle.c FP64 forward + backward on the same {ev} events, "
+                                          f"one step in {cdt:.2f} s, 1 thread"}
+    return line
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -185,6 +273,9 @@ def main():
         sys.exit(2)
     if args.form == "batched":
         print(json.dumps(batched(args)), flush=True)
+        return
+    if args.form == "adjoint":
+        print(json.dumps(adjoint_form(args)), flush=True)
         return
     import numpy as np
     import snncuda_amd as sa

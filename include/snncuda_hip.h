@@ -185,6 +185,42 @@ int snn_batch_run_until(snn_batch* batch, uint64_t inclusive_tick);
  * totals, arrays of n_nets. delivered needs track_pre_count; either pointer may be NULL. */
 int snn_batch_net_counts(snn_batch* batch, uint64_t* delivered, uint64_t* fired);
 
+/* Recording of a run for the fixed-spike adjoint (no counterpart in the reference). Switched on
+ * before the first tick, the step keeps, for every fire of the raster, the index of the event the
+ * neuron fired on among its ordered events of that tick, and keeps every delivered inject (neuron,
+ * tick, scheduler poll tick at the inject call, weight) in delivery order. */
+int snn_net_record_fire_events(snn_net* net, int enabled);
+int snn_net_read_fire_events(snn_net* net, uint64_t* n_out, uint32_t* event_index, uint64_t cap);
+int snn_net_read_inject_log(snn_net* net, uint64_t* n_out, uint32_t* neuron, uint64_t* tick, uint64_t* stage,
+                            float* weight, uint64_t cap);
+
+/* Fixed-spike adjoint (no counterpart in the reference, which has no gradients). The subthreshold
+ * receptor / dendrite / soma dynamics of the step (reference src/runtime/SynapseDendriteProcessor.cpp:175-229)
+ * with every event time and every reset held at those of a recorded run (ticks [0, ticks)):
+ *   c_k(t) = exp(-1/tau[r_k]) c_k(t-1), then each event of tick t in processing order:
+ *   c_comp += w * gain[r] * sign[r], r_comp = r, and c = 0 on the event the neuron fired on;
+ *   trace v(t) = sum_k alpha[k] c_k(t)   (alpha = 1: the reference membrane sum, :287-292)
+ * Parameters: synapse weights (connectome order) and params[8] = alpha[4] (soma, basal, apical,
+ * inhibitory), tau[4] (AMPA, NMDA, GABA_A, GABA_B; reference values 5, 100, 10, 150).
+ * snn_adjoint_create builds every neuron's event list from a run recorded with
+ * snn_net_record_fire_events and the same connectome. Device pointers (d_*) live on the device that
+ * was current at create; calls synchronize the adjoint's stream before returning.
+ * snn_adjoint_forward writes the trace [ticks][n_neurons] and keeps checkpoints every 64 ticks;
+ * snn_adjoint_backward takes dL/dtrace [ticks][n_neurons] for the last forward's parameters,
+ * recomputes each 64-tick segment from its checkpoint and writes dL/dweight (device, n_synapses) and
+ * dL/dparams (host, 8 doubles). */
+typedef struct snn_adjoint snn_adjoint;
+int snn_adjoint_create(snn_net* run, const snn_net_desc* desc, uint64_t ticks, snn_adjoint** out);
+int snn_adjoint_destroy(snn_adjoint* adj);
+int snn_adjoint_sizes(snn_adjoint* adj, uint64_t* n_neurons, uint64_t* n_synapses, uint64_t* ticks,
+                      uint64_t* n_events);
+/* The event lists (host copies): ptr[n+1]; per event tick, synapse (UINT32_MAX for an inject),
+ * inject weight, reset flag. Any pointer may be NULL. */
+int snn_adjoint_read_events(snn_adjoint* adj, uint64_t* ptr, uint32_t* tick, uint32_t* synapse, float* weight,
+                            uint8_t* reset);
+int snn_adjoint_forward(snn_adjoint* adj, const float* d_weight, const float* params, float* d_trace);
+int snn_adjoint_backward(snn_adjoint* adj, const float* d_grad_trace, float* d_grad_weight, double* grad_params);
+
 /* Mean duration (ms) of each step kernel over the last run, measured with HIP events on the
  * handle's stream when profiling is enabled (snn_net_set_profiling). names receives a
  * comma-separated list. Returns the number of kernels written. */

@@ -320,6 +320,151 @@ Result run_resident_propagation(const Connectome& connectome, const std::vector<
     return result;
 }
 
+// CudaInferenceSession::run_batch (reference include/snncuda/backends/CudaBackend.h:97-126,
+// src/backends/cuda/CudaBackend.cu:1692-1723) with the reference CPU semantics.
+//
+// Stateless (options.reset_state_between_samples, the reference default): every sample runs on a
+// fresh copy of the network, all copies stepped together as one batch (snn_batch_*): sample s's
+// inputs are injects (neuron, tick, weight) in input order (ticks above max_steps and unknown ids
+// skipped, as the reference skips them, CudaBackend.cu:2316-2324), then ticks 0..max_steps run.
+// Two inputs to one neuron at one tick stay two injects, as on the CPU (the reference CUDA path sums
+// them into one, :2325-2330). Per sample: delivered and fired totals (injects included, as the CPU
+// counts them), the readout neurons' spike counts, ticks_processed, processed/scheduled synaptic
+// events and fired_appends. readout_scores stays empty, as on the reference's stateless path.
+//
+// Stateful (reset_state_between_samples false): one network keeps its state across samples and
+// calls, and the clock continues: a sample's input tick t lands at (first tick of the sample) + t and
+// the sample runs max_steps + 1 ticks; its counts are the increments over the sample. (The
+// reference CUDA path restarts its tick counter at 0 for every sample while keeping state.)
+template <class BatchResult, class Sample, class Options>
+class InferenceRunner {
+public:
+    explicit InferenceRunner(PackedConnectome p) : p_(std::move(p)) {}
+
+    BatchResult run(const std::vector<Sample>& samples, const Options& options) {
+        BatchResult batch{};
+        batch.executed = true;
+        const auto start = std::chrono::steady_clock::now();
+        if (!samples.empty() && !p_.threshold.empty()) {
+            if (options.reset_state_between_samples) run_stateless(samples, options, batch);
+            else run_stateful(samples, options, batch);
+        } else {
+            batch.samples.resize(samples.size());
+            for (auto& r : batch.samples) r.readout_spike_counts.assign(options.readout_neurons.size(), 0);
+        }
+        batch.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        return batch;
+    }
+
+    void reset() { state_ = Network(); }
+
+private:
+    template <class Result>
+    void fill_debug(Result& r, const Options& options, std::uint64_t synaptic, std::uint64_t fired) const {
+        r.debug.ticks_processed = static_cast<std::uint64_t>(options.max_steps) + 1;
+        r.debug.processed_events = synaptic;
+        r.debug.scheduled_events = synaptic;
+        r.debug.dendritic_integrations = synaptic;
+        r.debug.fired_appends = fired;
+    }
+
+    void run_stateless(const std::vector<Sample>& samples, const Options& options, BatchResult& batch) {
+        const std::size_t n = p_.threshold.size(), e = p_.source.size();
+        // replicas per launch group: the union stays below the device's 2^32 neuron / synapse limits
+        const std::size_t per = std::max<std::size_t>(
+            1, std::min<std::size_t>((std::size_t)0xF0000000ull / std::max<std::size_t>(n, 1),
+                                     (std::size_t)0xF0000000ull / std::max<std::size_t>(e, 1)));
+        batch.samples.resize(samples.size());
+        const snn_net_desc d = p_.desc();
+        for (std::size_t first = 0; first < samples.size(); first += per) {
+            const std::size_t m = std::min(per, samples.size() - first);
+            std::vector<snn_net_desc> descs(m, d);
+            snn_options o{};
+            o.track_pre_count = 1;
+            snn_batch* b = nullptr;
+            check(snn_batch_create(descs.data(), m, &o, &b), "snn_batch_create");
+            struct Guard {
+                snn_batch* b;
+                ~Guard() { snn_batch_destroy(b); }
+            } guard{b};
+            std::vector<std::uint32_t> net, neuron;
+            std::vector<std::uint64_t> tick;
+            std::vector<float> weight;
+            std::vector<std::uint64_t> injected(m, 0);
+            for (std::size_t s = 0; s < m; ++s)
+                for (const auto& in : samples[first + s].inputs) {
+                    if (in.tick > options.max_steps) continue;
+                    const auto it = p_.index.find(static_cast<std::uint64_t>(in.neuron));
+                    if (it == p_.index.end()) continue;
+                    net.push_back(static_cast<std::uint32_t>(s));
+                    neuron.push_back(it->second);
+                    tick.push_back(in.tick);
+                    weight.push_back(in.weight);
+                    ++injected[s];
+                }
+            check(snn_batch_inject(b, net.data(), neuron.data(), tick.data(), weight.data(), net.size()),
+                  "snn_batch_inject");
+            check(snn_batch_run_until(b, options.max_steps), "snn_batch_run_until");
+            std::vector<std::uint64_t> delivered(m), fired(m), spikes(m * n);
+            check(snn_batch_net_counts(b, delivered.data(), fired.data()), "snn_batch_net_counts");
+            snn_net* u = nullptr;
+            check(snn_batch_network(b, &u), "snn_batch_network");
+            check(snn_net_read_neurons(u, nullptr, nullptr, nullptr, spikes.data(), nullptr, nullptr, nullptr),
+                  "snn_net_read_neurons");
+            for (std::size_t s = 0; s < m; ++s) {
+                auto& r = batch.samples[first + s];
+                r.delivered_spikes = delivered[s];
+                r.fired_spikes = fired[s];
+                for (const auto id : options.readout_neurons) {
+                    const auto it = p_.index.find(static_cast<std::uint64_t>(id));
+                    r.readout_spike_counts.push_back(it == p_.index.end() ? 0 : spikes[s * n + it->second]);
+                }
+                fill_debug(r, options, delivered[s] - injected[s], fired[s]);
+            }
+        }
+    }
+
+    void run_stateful(const std::vector<Sample>& samples, const Options& options, BatchResult& batch) {
+        if (!state_.get()) {
+            snn_options o{};
+            o.track_pre_count = 1;
+            state_ = Network(p_.desc(), &o);
+        }
+        const std::size_t n = p_.threshold.size();
+        std::vector<std::uint64_t> before(n), after(n);
+        for (const auto& sample : samples) {
+            const Counts c0 = state_.counts();
+            check(snn_net_read_neurons(state_.get(), nullptr, nullptr, nullptr, before.data(), nullptr, nullptr, nullptr),
+                  "snn_net_read_neurons");
+            std::uint64_t injected = 0;
+            for (const auto& in : sample.inputs) {
+                if (in.tick > options.max_steps) continue;
+                const auto it = p_.index.find(static_cast<std::uint64_t>(in.neuron));
+                if (it == p_.index.end()) continue;
+                state_.inject(it->second, c0.next_tick + in.tick, in.weight);
+                ++injected;
+            }
+            state_.run_until(c0.next_tick + options.max_steps);
+            const Counts c1 = state_.counts();
+            check(snn_net_read_neurons(state_.get(), nullptr, nullptr, nullptr, after.data(), nullptr, nullptr, nullptr),
+                  "snn_net_read_neurons");
+            typename decltype(batch.samples)::value_type r{};
+            r.delivered_spikes = c1.delivered - c0.delivered;
+            r.fired_spikes = c1.fired - c0.fired;
+            for (const auto id : options.readout_neurons) {
+                const auto it = p_.index.find(static_cast<std::uint64_t>(id));
+                r.readout_spike_counts.push_back(it == p_.index.end() ? 0 : after[it->second] - before[it->second]);
+            }
+            fill_debug(r, options, r.delivered_spikes - injected, r.fired_spikes);
+            batch.samples.push_back(std::move(r));
+        }
+    }
+
+    using Counts = Network::Counts;
+    PackedConnectome p_;
+    Network state_;
+};
+
 }  // namespace snncuda_hip
 
 #endif

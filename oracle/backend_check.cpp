@@ -10,6 +10,10 @@
 //       initially_active, max_steps). Every CudaPropagationResult vector must equal the CPU state
 //       bit for bit, and the counters must agree. Exit 0 on agreement, 1 on a mismatch, 3 without a
 //       device.
+//   backend_check session <net.bin> <max_steps> <samples>
+//       CudaInferenceSession::run_batch: the reference's session tests restated on the LTP pair, then
+//       samples built from the file's injects against fresh CPU NetworkPropagator runs (stateless)
+//       and against one CPU propagator with a continuing clock (stateful).
 //   backend_check pack <net.bin>
 //       Host-only: snncuda_hip::pack of the Connectome must reproduce the file's arrays exactly
 //       (ids mapped back to indices, empty code lists kept empty), and a synapse naming an unknown
@@ -19,6 +23,7 @@
 #include "snncuda/runtime/NetworkPropagator.h"
 #include "snncuda_hip.hpp"
 
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -171,6 +176,135 @@ int cmd_run(const netfile::Net& t, std::uint32_t max_steps) {
     return mismatches ? 1 : 0;
 }
 
+// The LTP pair of the reference's CUDA tests (tests/cuda_synapse_dendrite_tests.cpp:20-80): input
+// (threshold 1.0) -> target (threshold 0.4), weight 0.5, delay 1, Basal, AMPA, plastic.
+snncuda::declarative::Connectome ltp_pair() {
+    snncuda::declarative::Connectome c;
+    c.neurons.resize(2);
+    c.neurons[0].id = 1;
+    c.neurons[0].params.threshold = 1.0F;
+    c.neurons[1].id = 2;
+    c.neurons[1].params.threshold = 0.4F;
+    c.synapses.resize(1);
+    c.synapses[0].id = 1;
+    c.synapses[0].source = 1;
+    c.synapses[0].target = 2;
+    c.synapses[0].weight = 0.5F;
+    return c;
+}
+
+void require(bool ok, const char* what) {
+    if (!ok) {
+        std::fprintf(stderr, "failed: %s\n", what);
+        ++mismatches;
+    }
+}
+
+// The reference's session tests restated (cuda_synapse_dendrite_tests.cpp:238-303,421-463), then
+// every sample of a batch built from a network file against a fresh CPU NetworkPropagator run, and
+// the stateful mode against one CPU propagator whose clock continues across samples.
+int cmd_session(const netfile::Net& t, std::uint32_t max_steps, std::uint32_t n_samples) {
+    using namespace snncuda::backends;
+    const CudaBackend backend;
+    if (!backend.available()) {
+        std::printf("backend_check: %s\n", backend.availability_status().c_str());
+        return 3;
+    }
+    {
+        const auto c = ltp_pair();
+        const auto first = backend.run_resident_propagation(c, {1}, 1);
+        const auto second = backend.run_resident_propagation(c, {1}, 1);
+        CudaInferenceSession session(c);
+        CudaInferenceOptions options;
+        options.max_steps = 1;
+        options.readout_neurons = {2};
+        const auto batch = session.run_batch({{.inputs = {{.neuron = 1}}}, {.inputs = {{.neuron = 1}}}, {.inputs = {}}},
+                                             options);
+        require(batch.executed && batch.samples.size() == 3, "session returns every sample");
+        require(batch.samples[0].delivered_spikes == first.delivered_spikes, "sample 0 delivered = single run");
+        require(batch.samples[0].fired_spikes == first.fired_spikes, "sample 0 fired = single run");
+        require(batch.samples[0].readout_spike_counts.size() == 1 &&
+                    batch.samples[0].readout_spike_counts[0] == first.final_neuron_spike_counts[1],
+                "sample 0 readout = single run");
+        require(batch.samples[1].delivered_spikes == second.delivered_spikes &&
+                    batch.samples[1].readout_spike_counts[0] == second.final_neuron_spike_counts[1],
+                "sample 1 = single run");
+        require(batch.samples[2].delivered_spikes == 0 && batch.samples[2].fired_spikes == 0 &&
+                    batch.samples[2].readout_spike_counts[0] == 0,
+                "empty sample is empty");
+        const auto weighted = session.run_batch(
+            {{.inputs = {{.neuron = 1, .weight = 0.5F}}},
+             {.inputs = {{.neuron = 1, .weight = 1.0F}}},
+             {.inputs = {{.neuron = 1, .weight = 0.4F}, {.neuron = 1, .weight = 0.6F}}}},
+            options);
+        require(weighted.samples[0].delivered_spikes == 1 && weighted.samples[0].fired_spikes == 0 &&
+                    weighted.samples[0].readout_spike_counts[0] == 0,
+                "subthreshold weighted input");
+        require(weighted.samples[1].fired_spikes == 2 && weighted.samples[1].readout_spike_counts[0] == 1,
+                "threshold weighted input fires input and readout");
+        require(weighted.samples[2].fired_spikes == 2 && weighted.samples[2].readout_spike_counts[0] == 1,
+                "accumulated weighted inputs fire input and readout");
+        std::vector<CudaInferenceSample> many(100, CudaInferenceSample{.inputs = {{.neuron = 1}}});
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < 100; ++k) require(backend.run_resident_propagation(c, {1}, 1).executed, "repeated run");
+        const double repeated = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const auto persistent = session.run_batch(many, options);
+        require(persistent.samples.size() == 100 && persistent.elapsed_seconds > 0.0, "100-sample batch");
+        require(persistent.elapsed_seconds < repeated, "one batch beats 100 separate runs");
+        std::printf("session: 100 separate runs %.4f s, one batch %.4f s\n", repeated, persistent.elapsed_seconds);
+    }
+    // samples from the file's injects: inject q goes to sample q % n_samples (ticks <= max_steps)
+    const auto c = netfile::connectome(t);
+    std::vector<CudaInferenceSample> samples(n_samples);
+    for (std::size_t q = 0; q < t.cmds.size(); ++q) {
+        const auto& cmd = t.cmds[q];
+        if (cmd.kind != SNN_CMD_INJECT) continue;
+        samples[q % n_samples].inputs.push_back(
+            {.neuron = cmd.neuron + 1ull, .weight = cmd.weight, .tick = static_cast<std::uint32_t>(cmd.tick)});
+    }
+    CudaInferenceOptions options;
+    options.max_steps = max_steps;
+    for (std::uint64_t i = 0; i < t.n; ++i) options.readout_neurons.push_back(i + 1);
+    CudaInferenceSession session(c);
+    const auto batch = session.run_batch(samples, options);
+    for (std::uint32_t s = 0; s < n_samples; ++s) {
+        snncuda::runtime::MemoryNeuronStateStore store;
+        snncuda::runtime::NetworkPropagator cpu(c, store, t.n ? t.n : 1, 4096);
+        for (const auto& in : samples[s].inputs)
+            if (in.tick <= max_steps) cpu.inject(in.neuron, in.tick, in.weight);
+        cpu.run_until(max_steps);
+        const auto& r = batch.samples[s];
+        expect_eq("session delivered", s, r.delivered_spikes, cpu.delivered_spike_count());
+        expect_eq("session fired", s, r.fired_spikes, cpu.fired_spike_count());
+        for (std::uint64_t i = 0; i < t.n; ++i) expect_eq("session readout", i, r.readout_spike_counts[i], cpu.spike_count(i + 1));
+    }
+    // stateful: one propagator, the clock continues across samples
+    options.reset_state_between_samples = false;
+    const auto seq = session.run_batch(samples, options);
+    {
+        snncuda::runtime::MemoryNeuronStateStore store;
+        snncuda::runtime::NetworkPropagator cpu(c, store, t.n ? t.n : 1, 4096);
+        std::uint64_t base = 0;
+        std::vector<std::uint64_t> before(t.n);
+        for (std::uint32_t s = 0; s < n_samples; ++s) {
+            const auto d0 = cpu.delivered_spike_count(), f0 = cpu.fired_spike_count();
+            for (std::uint64_t i = 0; i < t.n; ++i) before[i] = cpu.spike_count(i + 1);
+            for (const auto& in : samples[s].inputs)
+                if (in.tick <= max_steps) cpu.inject(in.neuron, base + in.tick, in.weight);
+            cpu.run_until(base + max_steps);
+            const auto& r = seq.samples[s];
+            expect_eq("stateful delivered", s, r.delivered_spikes, cpu.delivered_spike_count() - d0);
+            expect_eq("stateful fired", s, r.fired_spikes, cpu.fired_spike_count() - f0);
+            for (std::uint64_t i = 0; i < t.n; ++i)
+                expect_eq("stateful readout", i, r.readout_spike_counts[i], cpu.spike_count(i + 1) - before[i]);
+            base += max_steps + 1ull;
+        }
+    }
+    std::printf("backend_check session %s: %u samples, max_steps=%u, mismatches=%d\n", mismatches ? "FAILED" : "ok",
+                n_samples, max_steps, mismatches);
+    return mismatches ? 1 : 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -179,10 +313,15 @@ int main(int argc, char** argv) {
         if (argc == 4 && std::strcmp(argv[1], "run") == 0) {
             return cmd_run(netfile::read(argv[2]), static_cast<std::uint32_t>(std::stoul(argv[3])));
         }
+        if (argc == 5 && std::strcmp(argv[1], "session") == 0) {
+            return cmd_session(netfile::read(argv[2]), static_cast<std::uint32_t>(std::stoul(argv[3])),
+                               static_cast<std::uint32_t>(std::stoul(argv[4])));
+        }
     } catch (const std::exception& err) {
         std::fprintf(stderr, "backend_check failed: %s\n", err.what());
         return 1;
     }
-    std::fprintf(stderr, "usage: %s run <net.bin> <max_steps> | pack <net.bin>\n", argv[0]);
+    std::fprintf(stderr, "usage: %s run <net.bin> <max_steps> | pack <net.bin> | session <net.bin> <max_steps> <samples>\n",
+                 argv[0]);
     return 2;
 }

@@ -106,6 +106,15 @@ def _load() -> ctypes.CDLL:
     L.snn_net_autotune.argtypes = [P, ctypes.c_uint32]
     L.snn_net_tune_log.argtypes = [P, ctypes.c_char_p, U64]
     L.snn_batch_create.argtypes = [P, U64, ctypes.POINTER(Options), ctypes.POINTER(P)]
+    L.snn_net_record_fire_events.argtypes = [P, I]
+    L.snn_net_read_fire_events.argtypes = [P, P, P, U64]
+    L.snn_net_read_inject_log.argtypes = [P, P, P, P, P, P, U64]
+    L.snn_adjoint_create.argtypes = [P, ctypes.POINTER(_Desc), U64, ctypes.POINTER(P)]
+    L.snn_adjoint_destroy.argtypes = [P]
+    L.snn_adjoint_sizes.argtypes = [P, P, P, P, P]
+    L.snn_adjoint_read_events.argtypes = [P] * 6
+    L.snn_adjoint_forward.argtypes = [P, P, P, P]
+    L.snn_adjoint_backward.argtypes = [P, P, P, P]
     L.snn_batch_create_ei.argtypes = [P, U64, ctypes.POINTER(Options), ctypes.POINTER(P)]
     L.snn_batch_destroy.argtypes = [P]
     L.snn_batch_network.argtypes = [P, ctypes.POINTER(P)]
@@ -120,7 +129,9 @@ def _load() -> ctypes.CDLL:
                  "snn_net_read_synapses", "snn_net_read_plasticity", "snn_net_set_profiling",
                  "snn_net_kernel_times", "snn_net_get_schedule", "snn_net_set_schedule", "snn_net_autotune",
                  "snn_net_tune_log", "snn_batch_create", "snn_batch_create_ei", "snn_batch_destroy",
-                 "snn_batch_network",
+                 "snn_batch_network", "snn_net_record_fire_events", "snn_net_read_fire_events",
+                 "snn_net_read_inject_log", "snn_adjoint_create", "snn_adjoint_destroy", "snn_adjoint_sizes",
+                 "snn_adjoint_read_events", "snn_adjoint_forward", "snn_adjoint_backward",
                  "snn_batch_offsets", "snn_batch_inject", "snn_batch_run_until", "snn_batch_net_counts"):
         getattr(L, name).restype = I
     return L
@@ -298,6 +309,28 @@ class Network:
         rest (ltd); ltp + ltd == synapses()["upd_count"]."""
         o = dict(ltp=np.zeros(self.e, np.uint64), ltd=np.zeros(self.e, np.uint64))
         _check(lib().snn_net_read_plasticity(self._h, _p(o["ltp"]), _p(o["ltd"])))
+        return o
+
+    # recording for the fixed-spike adjoint
+    def record_fire_events(self, on: bool = True) -> None:
+        """Keep each fire's event index and every delivered inject (switch on before the first tick)."""
+        _check(lib().snn_net_record_fire_events(self._h, int(on)))
+
+    def fire_events(self) -> np.ndarray:
+        n = ctypes.c_uint64()
+        _check(lib().snn_net_read_fire_events(self._h, ctypes.byref(n), None, 0))
+        out = np.zeros(n.value, np.uint32)
+        _check(lib().snn_net_read_fire_events(self._h, ctypes.byref(n), _p(out), out.shape[0]))
+        return out
+
+    def inject_log(self) -> Dict[str, np.ndarray]:
+        n = ctypes.c_uint64()
+        _check(lib().snn_net_read_inject_log(self._h, ctypes.byref(n), None, None, None, None, 0))
+        m = n.value
+        o = dict(neuron=np.zeros(m, np.uint32), tick=np.zeros(m, np.uint64), stage=np.zeros(m, np.uint64),
+                 weight=np.zeros(m, np.float32))
+        _check(lib().snn_net_read_inject_log(self._h, ctypes.byref(n), *[_p(o[k]) for k in ("neuron", "tick", "stage",
+                                                                                              "weight")], m))
         return o
 
     # launch schedule and autotuner (results are identical under every schedule)

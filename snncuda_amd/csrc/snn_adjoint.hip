@@ -1,0 +1,562 @@
+// Fixed-spike adjoint of the per-tick step's subthreshold dynamics (no counterpart in the
+// reference, which has no gradients).
+//
+// Input: a recorded run of the plain step (snn_net_record_fire_events): its raster in processing
+// order, the index of each firing event within its target's ordered events of the tick, and every
+// delivered inject. From these and the connectome the host builds each neuron's events of ticks
+// [0, T) in the exact processing order of the step (reference src/runtime/NetworkPropagator.cpp:78-132
+// with SpikeScheduler's insertion order): within tick T, injects of stage P sit after the fires of
+// tick P-1 and before those of tick P, and the events of one fire follow its outgoing synapses in
+// connectome order and each synapse's code offsets in list order. The event a neuron fired on is
+// its reset event.
+//
+// Model per neuron, compartments k = 0..3 (reference src/runtime/SynapseDendriteProcessor.cpp:175-229
+// with the reset at the recorded events instead of a threshold test):
+//   c_k(t) = A[r_k] * c_k(t-1), A[r] = exp(-1/tau_r), r_k the receptor of compartment k's last input
+//   each event of tick t in order: c_comp += w * gain[r] * sign[r]; r_comp = r; reset event: c = 0
+//   v(t) = sum_k alpha_k c_k(t)                       (the somatic trace; alpha = 1 is the reference
+//                                                      membrane sum, :287-292)
+// Forward: one thread per neuron walks its events tick by tick, writes v [T][N] and checkpoints
+// (c, r, event position) after every 64th tick. Backward: per 64-tick segment, from the last, the
+// thread recomputes the segment from its checkpoint into a per-neuron scratch and runs the adjoint
+// recursion back through it:
+//   lambda_k += dL/dv(t) * alpha_k; dL/dalpha_k += dL/dv(t) * c_k(t)
+//   events of tick t in reverse: reset -> lambda = 0; dL/dw_syn += lambda_comp * gain * sign
+//   dL/dtau_r += lambda_k c_k(t-1) A[r] / tau_r^2 over k with r_k(t-1) = r; lambda_k *= A[r_k(t-1)]
+// A synapse's events all belong to its target, so the thread owning a neuron is the only writer of
+// its incoming synapses' weight gradients.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/snncuda_hip.h"
+
+void snn_internal_set_error(const std::string& msg);
+
+namespace {
+
+struct AdjError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void acheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw AdjError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int afail(int code, const std::string& msg) {
+    snn_internal_set_error(msg);
+    return code;
+}
+
+template <typename F>
+int aguarded(F&& f) {
+    try {
+        return f();
+    } catch (const AdjError& e) {
+        return afail(SNN_EDEVICE, e.what());
+    } catch (const std::invalid_argument& e) {
+        return afail(SNN_EINVAL, e.what());
+    } catch (const std::bad_alloc&) {
+        return afail(SNN_EINVAL, "host allocation failed");
+    } catch (const std::exception& e) {
+        return afail(SNN_EDEVICE, e.what());
+    }
+}
+
+constexpr uint32_t kSeg = 64;  // checkpoint interval (ticks)
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kBlock = 256;
+
+// One event: the synapse (connectome index) or, for an external inject, the weight's bits; tick;
+// meta = compartment | receptor << 2 | external << 4 | reset << 5.
+struct AdjEvent {
+    uint32_t ref;
+    uint16_t tick;
+    uint8_t meta;
+    uint8_t pad;
+};
+
+struct AdjParams {
+    float alpha[4];
+    float A[4];   // exp(-1/tau_r)
+    float dA[4];  // d A / d tau_r = A / tau_r^2
+};
+
+__constant__ float kGainSign[4] = {1.0f, 0.55f, -1.0f, -0.65f};
+
+struct AdjArgs {
+    uint32_t n, T, nseg;
+    const uint64_t* ptr;    // n + 1
+    const AdjEvent* ev;
+    const float* w;         // synapse weights, connectome order
+    float* trace;           // [T][n]
+    float* ck;              // [nseg][4][n] state after tick 64 s + 63
+    uint32_t* ckr;          // [nseg][n] receptor codes (2 bits per compartment)
+    uint64_t* ckp;          // [nseg + 1][n] event position at the start of segment s
+    float* scr;             // [64][4][n] segment recompute scratch
+    uint32_t* scr_r;        // [64][n]
+    const float* gtrace;    // [T][n]
+    float* gw;              // synapse weight gradients, connectome order
+    double* part;           // [n][8] per-neuron dL/dalpha, dL/dtau
+};
+
+__device__ __forceinline__ float ev_input(const AdjEvent& e, const float* w) {
+    const uint32_t r = (e.meta >> 2) & 3u;
+    const float wt = (e.meta & 16u) ? __uint_as_float(e.ref) : w[e.ref];
+    return wt * kGainSign[r];
+}
+
+// one tick of the forward dynamics for neuron i: decay, then the events of tick t from *p on
+__device__ __forceinline__ void adj_tick(const AdjArgs& a, const AdjParams& P, uint32_t t, uint64_t& p, uint64_t end,
+                                         float c[4], uint32_t& rk) {
+    if (t > 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c[k] *= P.A[(rk >> (2 * k)) & 3u];
+    }
+    for (; p < end; ++p) {
+        const AdjEvent e = a.ev[p];
+        if (e.tick != t) break;
+        const uint32_t k = e.meta & 3u;
+        const uint32_t r = (e.meta >> 2) & 3u;
+        const float u = ev_input(e, a.w);
+        c[0] = k == 0 ? c[0] + u : c[0];
+        c[1] = k == 1 ? c[1] + u : c[1];
+        c[2] = k == 2 ? c[2] + u : c[2];
+        c[3] = k == 3 ? c[3] + u : c[3];
+        rk = (rk & ~(3u << (2 * k))) | (r << (2 * k));
+        if (e.meta & 32u) c[0] = c[1] = c[2] = c[3] = 0.0f;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_adj_forward(AdjArgs a, AdjParams P) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t rk = 0;
+    uint64_t p = a.ptr[i];
+    const uint64_t end = a.ptr[i + 1];
+    a.ckp[i] = p;
+    for (uint32_t t = 0; t < a.T; ++t) {
+        adj_tick(a, P, t, p, end, c, rk);
+        a.trace[(uint64_t)t * a.n + i] = P.alpha[0] * c[0] + P.alpha[1] * c[1] + P.alpha[2] * c[2] + P.alpha[3] * c[3];
+        if ((t % kSeg) == kSeg - 1 || t + 1 == a.T) {
+            const uint32_t s = t / kSeg;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) a.ck[((uint64_t)s * 4 + k) * a.n + i] = c[k];
+            a.ckr[(uint64_t)s * a.n + i] = rk;
+            a.ckp[(uint64_t)(s + 1) * a.n + i] = p;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_adj_backward(AdjArgs a, AdjParams P) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    float lam[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float galpha[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float gtau[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t s = a.nseg; s-- > 0;) {
+        const uint32_t t0 = s * kSeg, t1 = min(a.T, t0 + kSeg);
+        // state before the segment
+        float c0[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        uint32_t r0 = 0;
+        if (s > 0) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) c0[k] = a.ck[((uint64_t)(s - 1) * 4 + k) * a.n + i];
+            r0 = a.ckr[(uint64_t)(s - 1) * a.n + i];
+        }
+        // recompute the segment from its checkpoint
+        {
+            float c[4] = {c0[0], c0[1], c0[2], c0[3]};
+            uint32_t rk = r0;
+            uint64_t p = a.ckp[(uint64_t)s * a.n + i];
+            const uint64_t end = a.ckp[(uint64_t)(s + 1) * a.n + i];
+            for (uint32_t t = t0; t < t1; ++t) {
+                adj_tick(a, P, t, p, end, c, rk);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) a.scr[((uint64_t)(t - t0) * 4 + k) * a.n + i] = c[k];
+                a.scr_r[(uint64_t)(t - t0) * a.n + i] = rk;
+            }
+        }
+        // adjoint recursion back through the segment
+        uint64_t p = a.ckp[(uint64_t)(s + 1) * a.n + i];
+        const uint64_t pbeg = a.ckp[(uint64_t)s * a.n + i];
+        for (uint32_t t = t1; t-- > t0;) {
+            const float g = a.gtrace[(uint64_t)t * a.n + i];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                galpha[k] += g * a.scr[((uint64_t)(t - t0) * 4 + k) * a.n + i];
+                lam[k] += g * P.alpha[k];
+            }
+            for (; p > pbeg; --p) {
+                const AdjEvent e = a.ev[p - 1];
+                if (e.tick != t) break;
+                if (e.meta & 32u) lam[0] = lam[1] = lam[2] = lam[3] = 0.0f;
+                if (!(e.meta & 16u)) {
+                    const uint32_t k = e.meta & 3u;
+                    const uint32_t r = (e.meta >> 2) & 3u;
+                    const float lk = k == 0 ? lam[0] : k == 1 ? lam[1] : k == 2 ? lam[2] : lam[3];
+                    a.gw[e.ref] += lk * kGainSign[r];
+                }
+            }
+            if (t > 0) {
+                float cp[4];
+                uint32_t rp;
+                if (t > t0) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) cp[k] = a.scr[((uint64_t)(t - 1 - t0) * 4 + k) * a.n + i];
+                    rp = a.scr_r[(uint64_t)(t - 1 - t0) * a.n + i];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) cp[k] = c0[k];
+                    rp = r0;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t r = (rp >> (2 * k)) & 3u;
+                    const float gt = lam[k] * cp[k] * P.dA[r];
+                    gtau[0] += r == 0 ? gt : 0.0f;
+                    gtau[1] += r == 1 ? gt : 0.0f;
+                    gtau[2] += r == 2 ? gt : 0.0f;
+                    gtau[3] += r == 3 ? gt : 0.0f;
+                    lam[k] *= P.A[r];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        a.part[(uint64_t)i * 8 + k] = galpha[k];
+        a.part[(uint64_t)i * 8 + 4 + k] = gtau[k];
+    }
+}
+
+// sum of the per-neuron partials in FP64, one block, fixed order
+__global__ void __launch_bounds__(kBlock) k_adj_reduce(const double* part, uint32_t n, double* out) {
+    __shared__ double sm[kBlock][8];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        for (int q = 0; q < 8; ++q) acc[q] += part[(uint64_t)i * 8 + q];
+    for (int q = 0; q < 8; ++q) sm[threadIdx.x][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        double s = 0.0;
+        for (int t = 0; t < kBlock; ++t) s += sm[t][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+struct DBuf {
+    void* p = nullptr;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+template <typename T>
+T* dalloc(std::vector<std::unique_ptr<DBuf>>& own, size_t count) {
+    auto b = std::make_unique<DBuf>();
+    acheck(hipMalloc(&b->p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+    T* p = (T*)b->p;
+    own.push_back(std::move(b));
+    return p;
+}
+
+}  // namespace
+
+struct snn_adjoint {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t n = 0, T = 0, nseg = 0;
+    uint64_t e = 0, nev = 0;
+    std::vector<std::unique_ptr<DBuf>> own;
+    AdjArgs a{};
+    AdjParams last{};
+    bool have_forward = false;
+    std::vector<uint64_t> h_ptr;
+    std::vector<AdjEvent> h_ev;
+    ~snn_adjoint() {
+        if (stream) {
+            (void)hipSetDevice(device);
+            (void)hipStreamSynchronize(stream);
+            own.clear();
+            (void)hipStreamDestroy(stream);
+        }
+    }
+};
+
+namespace {
+
+AdjParams make_params(const float* params) {
+    AdjParams P{};
+    for (int k = 0; k < 4; ++k) {
+        P.alpha[k] = params[k];
+        const double tau = params[4 + k];
+        if (!(tau > 0.0)) throw std::invalid_argument("time constants must be positive");
+        const double A = std::exp(-1.0 / tau);
+        P.A[k] = (float)A;
+        P.dA[k] = (float)(A / (tau * tau));
+    }
+    return P;
+}
+
+}  // namespace
+
+extern "C" {
+
+int snn_adjoint_create(snn_net* run, const snn_net_desc* desc, uint64_t ticks, snn_adjoint** out) {
+    if (!run || !desc || !out) return afail(SNN_EINVAL, "null argument");
+    *out = nullptr;
+    return aguarded([&]() -> int {
+        const uint64_t n = desc->n_neurons, e = desc->n_synapses;
+        if (ticks == 0 || ticks > 65535) return afail(SNN_EINVAL, "ticks must be in [1, 65535]");
+        if (n == 0 || n >= 0xFFFFFFFFull) return afail(SNN_EINVAL, "bad neuron count");
+        uint64_t delivered = 0, fired = 0, next = 0;
+        int rc = snn_net_counts(run, &delivered, &fired, &next);
+        if (rc) return rc;
+        if (next < ticks) return afail(SNN_EINVAL, "the run has not processed the requested ticks");
+        // raster, firing-event indices, per-tick fire counts, injects
+        uint64_t nf = 0;
+        rc = snn_net_read_raster(run, &nf, nullptr, 0);
+        if (rc) return rc;
+        std::vector<uint32_t> fire_neuron(nf), fire_evt(nf);
+        rc = snn_net_read_raster(run, &nf, fire_neuron.data(), nf);
+        if (rc) return rc;
+        uint64_t nf2 = 0;
+        rc = snn_net_read_fire_events(run, &nf2, fire_evt.data(), nf);
+        if (rc) return rc;
+        if (nf2 != nf) return afail(SNN_EINVAL, "raster and fire events differ in length");
+        std::vector<uint64_t> tick_fired(next);
+        rc = snn_net_read_tick_counts(run, 0, next, nullptr, tick_fired.data());
+        if (rc) return rc;
+        uint64_t ni = 0;
+        rc = snn_net_read_inject_log(run, &ni, nullptr, nullptr, nullptr, nullptr, 0);
+        if (rc) return rc;
+        std::vector<uint32_t> inj_neuron(ni);
+        std::vector<uint64_t> inj_tick(ni), inj_stage(ni);
+        std::vector<float> inj_w(ni);
+        rc = snn_net_read_inject_log(run, &ni, inj_neuron.data(), inj_tick.data(), inj_stage.data(), inj_w.data(), ni);
+        if (rc) return rc;
+
+        // outgoing synapses per source, connectome order
+        std::vector<uint64_t> optr(n + 1, 0);
+        for (uint64_t j = 0; j < e; ++j) {
+            if (desc->source[j] >= n || desc->target[j] >= n) throw std::invalid_argument("synapse endpoint out of range");
+            optr[desc->source[j] + 1]++;
+        }
+        for (uint64_t s = 0; s < n; ++s) optr[s + 1] += optr[s];
+        std::vector<uint32_t> oidx(e);
+        {
+            std::vector<uint64_t> fill(optr.begin(), optr.end() - 1);
+            for (uint64_t j = 0; j < e; ++j) oidx[fill[desc->source[j]]++] = (uint32_t)j;
+        }
+        static const uint32_t zero = 0;
+        auto codes_of = [&](uint64_t j, const uint32_t*& c) -> uint32_t {
+            if (desc->code_ptr && desc->code_ptr[j + 1] > desc->code_ptr[j]) {
+                c = desc->codes + desc->code_ptr[j];
+                return (uint32_t)(desc->code_ptr[j + 1] - desc->code_ptr[j]);
+            }
+            c = &zero;
+            return 1;
+        };
+
+        // events in emission order: [injects of stage P][fires of tick P] for P = 0, 1, ...
+        struct Raw {
+            uint32_t target, tick;
+            AdjEvent ev;
+        };
+        std::vector<Raw> raw;
+        std::vector<uint64_t> order(ni);
+        for (uint64_t q = 0; q < ni; ++q) order[q] = q;
+        std::stable_sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) { return inj_stage[x] < inj_stage[y]; });
+        uint64_t iq = 0, f = 0;
+        std::vector<uint64_t> fire_tick(nf);
+        for (uint64_t P = 0; P < next && P < ticks; ++P) {
+            for (; iq < ni && inj_stage[order[iq]] <= P; ++iq) {
+                const uint64_t q = order[iq];
+                if (inj_tick[q] >= ticks) continue;
+                AdjEvent ev{};
+                std::memcpy(&ev.ref, &inj_w[q], 4);
+                ev.tick = (uint16_t)inj_tick[q];
+                ev.meta = (uint8_t)(0u | (0u << 2) | 16u);  // soma, AMPA, external
+                raw.push_back({inj_neuron[q], (uint32_t)inj_tick[q], ev});
+            }
+            const uint64_t fend = f + tick_fired[P];
+            if (fend > nf) return afail(SNN_EINVAL, "raster shorter than the fire counts");
+            for (; f < fend; ++f) {
+                fire_tick[f] = P;
+                const uint32_t s = fire_neuron[f];
+                for (uint64_t q = optr[s]; q < optr[s + 1]; ++q) {
+                    const uint32_t j = oidx[q];
+                    const uint32_t* c;
+                    const uint32_t nc = codes_of(j, c);
+                    for (uint32_t k = 0; k < nc; ++k) {
+                        const uint64_t lag = std::max<uint64_t>(1, (uint64_t)desc->delay[j] + c[k]);
+                        const uint64_t T = P + lag;
+                        if (T >= ticks) continue;
+                        AdjEvent ev{};
+                        ev.ref = j;
+                        ev.tick = (uint16_t)T;
+                        ev.meta = (uint8_t)((desc->compartment[j] & 3u) | ((desc->receptor[j] & 3u) << 2));
+                        raw.push_back({desc->target[j], (uint32_t)T, ev});
+                    }
+                }
+            }
+        }
+        // per neuron, ordered by tick, emission order within a tick
+        std::unique_ptr<snn_adjoint> adj(new snn_adjoint());
+        adj->h_ptr.assign(n + 1, 0);
+        for (const Raw& r : raw) adj->h_ptr[r.target + 1]++;
+        for (uint64_t i = 0; i < n; ++i) adj->h_ptr[i + 1] += adj->h_ptr[i];
+        adj->h_ev.resize(raw.size());
+        std::vector<uint32_t> ev_tick(raw.size());
+        {
+            std::vector<uint64_t> fill(adj->h_ptr.begin(), adj->h_ptr.end() - 1);
+            for (const Raw& r : raw) {
+                const uint64_t at = fill[r.target]++;
+                adj->h_ev[at] = r.ev;
+                ev_tick[at] = r.tick;
+            }
+        }
+        raw.clear();
+        raw.shrink_to_fit();
+        std::vector<AdjEvent> tmp;
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint64_t lo = adj->h_ptr[i], hi = adj->h_ptr[i + 1];
+            std::vector<uint64_t> idx(hi - lo);
+            for (uint64_t q = 0; q < hi - lo; ++q) idx[q] = lo + q;
+            std::stable_sort(idx.begin(), idx.end(), [&](uint64_t x, uint64_t y) { return ev_tick[x] < ev_tick[y]; });
+            tmp.resize(hi - lo);
+            for (uint64_t q = 0; q < hi - lo; ++q) tmp[q] = adj->h_ev[idx[q]];
+            std::copy(tmp.begin(), tmp.end(), adj->h_ev.begin() + lo);
+        }
+        // reset events: fire f of neuron x at tick P happened on x's event fire_evt[f] of tick P
+        {
+            for (uint64_t q = 0; q < nf; ++q) {
+                if (fire_tick[q] >= ticks) continue;
+                const uint32_t x = fire_neuron[q];
+                const uint64_t lo = adj->h_ptr[x], hi = adj->h_ptr[x + 1];
+                // first event of tick P (binary search on the sorted ticks)
+                uint64_t a = lo, b = hi;
+                while (a < b) {
+                    const uint64_t m = (a + b) / 2;
+                    if (adj->h_ev[m].tick < fire_tick[q]) a = m + 1; else b = m;
+                }
+                const uint64_t at = a + fire_evt[q];
+                if (at >= hi || adj->h_ev[at].tick != fire_tick[q])
+                    return afail(SNN_EINVAL, "a recorded fire does not match its neuron's events");
+                adj->h_ev[at].meta |= 32u;
+            }
+        }
+        // device side
+        adj->n = (uint32_t)n;
+        adj->e = e;
+        adj->T = (uint32_t)ticks;
+        adj->nseg = (uint32_t)((ticks + kSeg - 1) / kSeg);
+        adj->nev = adj->h_ev.size();
+        int dev = 0;
+        acheck(hipGetDevice(&dev), "hipGetDevice");
+        adj->device = dev;
+        acheck(hipStreamCreate(&adj->stream), "hipStreamCreate");
+        AdjArgs& A = adj->a;
+        A.n = adj->n;
+        A.T = adj->T;
+        A.nseg = adj->nseg;
+        uint64_t* dptr = dalloc<uint64_t>(adj->own, n + 1);
+        AdjEvent* dev_ev = dalloc<AdjEvent>(adj->own, adj->nev);
+        acheck(hipMemcpy(dptr, adj->h_ptr.data(), (n + 1) * 8, hipMemcpyHostToDevice), "upload");
+        if (adj->nev) acheck(hipMemcpy(dev_ev, adj->h_ev.data(), adj->nev * sizeof(AdjEvent), hipMemcpyHostToDevice), "upload");
+        A.ptr = dptr;
+        A.ev = dev_ev;
+        A.ck = dalloc<float>(adj->own, (uint64_t)adj->nseg * 4 * n);
+        A.ckr = dalloc<uint32_t>(adj->own, (uint64_t)adj->nseg * n);
+        A.ckp = dalloc<uint64_t>(adj->own, (uint64_t)(adj->nseg + 1) * n);
+        A.scr = dalloc<float>(adj->own, (uint64_t)kSeg * 4 * n);
+        A.scr_r = dalloc<uint32_t>(adj->own, (uint64_t)kSeg * n);
+        A.part = dalloc<double>(adj->own, (uint64_t)n * 8);
+        *out = adj.release();
+        return SNN_OK;
+    });
+}
+
+int snn_adjoint_destroy(snn_adjoint* adj) {
+    delete adj;
+    return SNN_OK;
+}
+
+int snn_adjoint_sizes(snn_adjoint* adj, uint64_t* n_neurons, uint64_t* n_synapses, uint64_t* ticks, uint64_t* n_events) {
+    if (!adj) return afail(SNN_EINVAL, "null handle");
+    if (n_neurons) *n_neurons = adj->n;
+    if (n_synapses) *n_synapses = adj->e;
+    if (ticks) *ticks = adj->T;
+    if (n_events) *n_events = adj->nev;
+    return SNN_OK;
+}
+
+int snn_adjoint_read_events(snn_adjoint* adj, uint64_t* ptr, uint32_t* tick, uint32_t* synapse, float* weight,
+                            uint8_t* reset) {
+    if (!adj) return afail(SNN_EINVAL, "null handle");
+    if (ptr) std::copy(adj->h_ptr.begin(), adj->h_ptr.end(), ptr);
+    for (uint64_t q = 0; q < adj->nev; ++q) {
+        const AdjEvent& e = adj->h_ev[q];
+        const bool ext = (e.meta & 16u) != 0;
+        if (tick) tick[q] = e.tick;
+        if (synapse) synapse[q] = ext ? kNone : e.ref;
+        if (weight) {
+            float w = 0.0f;
+            if (ext) std::memcpy(&w, &e.ref, 4);
+            weight[q] = w;
+        }
+        if (reset) reset[q] = (e.meta & 32u) ? 1 : 0;
+    }
+    return SNN_OK;
+}
+
+int snn_adjoint_forward(snn_adjoint* adj, const float* d_weight, const float* params, float* d_trace) {
+    if (!adj || !params || !d_trace || (adj->e && !d_weight)) return afail(SNN_EINVAL, "null argument");
+    return aguarded([&]() -> int {
+        acheck(hipSetDevice(adj->device), "hipSetDevice");
+        const AdjParams P = make_params(params);
+        AdjArgs a = adj->a;
+        a.w = d_weight;
+        a.trace = d_trace;
+        const dim3 grid((adj->n + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_adj_forward, grid, dim3(kBlock), 0, adj->stream, a, P);
+        acheck(hipGetLastError(), "k_adj_forward");
+        acheck(hipStreamSynchronize(adj->stream), "adjoint forward");
+        adj->last = P;
+        adj->a.w = d_weight;
+        adj->have_forward = true;
+        return SNN_OK;
+    });
+}
+
+int snn_adjoint_backward(snn_adjoint* adj, const float* d_grad_trace, float* d_grad_weight, double* grad_params) {
+    if (!adj || !d_grad_trace || !grad_params || (adj->e && !d_grad_weight)) return afail(SNN_EINVAL, "null argument");
+    if (!adj->have_forward) return afail(SNN_EINVAL, "backward needs a forward first");
+    return aguarded([&]() -> int {
+        acheck(hipSetDevice(adj->device), "hipSetDevice");
+        AdjArgs a = adj->a;
+        a.gtrace = d_grad_trace;
+        a.gw = d_grad_weight;
+        if (adj->e) acheck(hipMemsetAsync(d_grad_weight, 0, adj->e * 4, adj->stream), "zero gradients");
+        const dim3 grid((adj->n + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_adj_backward, grid, dim3(kBlock), 0, adj->stream, a, adj->last);
+        acheck(hipGetLastError(), "k_adj_backward");
+        double* out = dalloc<double>(adj->own, 8);
+        hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(kBlock), 0, adj->stream, (const double*)a.part, adj->n, out);
+        acheck(hipMemcpyAsync(grad_params, out, 8 * sizeof(double), hipMemcpyDeviceToHost, adj->stream), "read");
+        acheck(hipStreamSynchronize(adj->stream), "adjoint backward");
+        adj->own.pop_back();
+        return SNN_OK;
+    });
+}
+
+}  // extern "C"

@@ -129,6 +129,8 @@ struct Step {
     uint32_t* tick_fired;
     uint32_t* raster;            // optional
     uint64_t raster_cap;
+    uint32_t* fire_evt;          // optional: per fire position g, the index of the firing event in the
+    uint32_t* raster_evt;        // target's ordered events of the tick; and per raster entry
 };
 
 SNN_HD uint32_t lower_bound_u32(const uint32_t* a, uint64_t lo, uint64_t hi, uint32_t v) {
@@ -376,6 +378,7 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
             SNN_ATOMIC_OR(&S.fire_bits[g >> 5], 1u << (g & 31u));
             S.fire_neuron[g] = x;
             S.fire_mult[g] = 0;
+            if (S.fire_evt) S.fire_evt[g] = i;
             if (fires == 0) first_g = g;
             ++fires;
         }

@@ -417,7 +417,10 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
                 S.firelog[idx] = S.fire_neuron[g];
                 S.firemult[idx] = S.fire_mult[g];
             }
-            if (rfits) S.raster[rfill + pos] = S.fire_neuron[g];
+            if (rfits) {
+                S.raster[rfill + pos] = S.fire_neuron[g];
+                if (S.raster_evt) S.raster_evt[rfill + pos] = S.fire_evt[g];
+            }
             ++pos;
         }
     }
@@ -848,6 +851,9 @@ struct snn_net {
     uint32_t max_nwg = 0;
     uint32_t tune_trial = 0;      // > 0: the next long enough run_until autotunes with chunks of this many ticks
     std::string tune_log;
+    // fire-event recording (fixed-spike adjoint input): every inject delivered, in delivery order
+    bool record = false;
+    std::vector<PendingInject> inj_log;
 
     ~snn_net() {
         if (device >= 0) (void)hipSetDevice(device);
@@ -1500,6 +1506,9 @@ std::unique_ptr<snn_net> generate_nets(const snn_ei_desc* g, uint64_t m, const s
 
 }  // namespace
 
+// the thread's last-error message, shared with the library's other translation units
+void snn_internal_set_error(const std::string& msg) { g_last_error = msg; }
+
 // ------------------------------------------------------------------------------------ C ABI
 
 extern "C" {
@@ -1560,6 +1569,59 @@ int snn_batch_create_ei(const snn_ei_desc* nets, uint64_t n_nets, const snn_opti
 
 int snn_net_destroy(snn_net* net) {
     delete net;
+    return SNN_OK;
+}
+
+int snn_net_record_fire_events(snn_net* net, int enabled) {
+    if (!net) return fail(SNN_EINVAL, "null handle");
+    if (net->next_tick != 0) return fail(SNN_EINVAL, "fire events can only be recorded from tick 0");
+    return guarded([&]() -> int {
+        check(hipSetDevice(net->device), "hipSetDevice");
+        net->record = enabled != 0;
+        if (net->record && !net->S.fire_evt) {
+            net->S.fire_evt = dev_alloc<uint32_t>(net->owned, net->S.ev_cap, false);
+            net->S.raster_evt = dev_alloc<uint32_t>(net->owned, net->S.raster_cap, false);
+        }
+        if (!net->record) {
+            net->S.fire_evt = nullptr;
+            net->S.raster_evt = nullptr;
+        }
+        return SNN_OK;
+    });
+}
+
+int snn_net_read_fire_events(snn_net* net, uint64_t* n_out, uint32_t* event_index, uint64_t cap) {
+    if (!net || !n_out) return fail(SNN_EINVAL, "null argument");
+    if (!net->S.raster_evt) return fail(SNN_EINVAL, "fire events were not recorded (snn_net_record_fire_events)");
+    return guarded([&]() -> int {
+        check(hipSetDevice(net->device), "hipSetDevice");
+        uint64_t tot[4];
+        check(hipMemcpy(tot, net->S.totals, sizeof(tot), hipMemcpyDeviceToHost), "read totals");
+        uint32_t ctl[4];
+        check(hipMemcpy(ctl, net->S.ctl, sizeof(ctl), hipMemcpyDeviceToHost), "read ctl");
+        *n_out = tot[3];
+        if (event_index && tot[3]) {
+            const uint64_t m = std::min(cap, tot[3]);
+            check(hipMemcpy(event_index, net->S.raster_evt, m * 4, hipMemcpyDeviceToHost), "read fire events");
+        }
+        if (ctl[3] & 1u) return fail(SNN_ECAPACITY, "raster capacity reached: only the first fires were recorded");
+        return SNN_OK;
+    });
+}
+
+int snn_net_read_inject_log(snn_net* net, uint64_t* n_out, uint32_t* neuron, uint64_t* tick, uint64_t* stage,
+                            float* weight, uint64_t cap) {
+    if (!net || !n_out) return fail(SNN_EINVAL, "null argument");
+    if (!net->record) return fail(SNN_EINVAL, "injects were not recorded (snn_net_record_fire_events)");
+    *n_out = net->inj_log.size();
+    const uint64_t m = std::min<uint64_t>(cap, net->inj_log.size());
+    for (uint64_t i = 0; i < m; ++i) {
+        const PendingInject& q = net->inj_log[i];
+        if (neuron) neuron[i] = q.neuron;
+        if (tick) tick[i] = q.tick;
+        if (stage) stage[i] = q.stage;
+        if (weight) weight[i] = q.weight;
+    }
     return SNN_OK;
 }
 
@@ -1788,6 +1850,7 @@ int snn_net_run_until(snn_net* net, uint64_t inclusive_tick) {
             return a.seq < b.seq;
         });
         net->pending.swap(later);
+        if (net->record) net->inj_log.insert(net->inj_log.end(), now.begin(), now.end());
         std::vector<uint32_t> itgt(now.size()), istage(now.size());
         std::vector<float> iw(now.size());
         std::vector<uint64_t> iptr(nt + 1, 0);

@@ -449,14 +449,17 @@ def union(nets: Sequence[Net], name: str = "union") -> Net:
 
 # --------------------------------------------------------------------------- adjoint workload
 
-def config4(seed: int = 0, n: int = 10_000, fan_in: int = 500, ticks: int = 500, threshold: float = 1.0,
-            p_in: float = 0.05, w_in: float = 0.6, delay_max: int = 20) -> Net:
+def config4(seed: int = 0, n: int = 10_000, fan_in: int = 500, ticks: int = 500, threshold: float = 40.0,
+            p_in: float = 0.01, w_in: float = 40.0, delay_max: int = 20) -> Net:
     """Network of the adjoint workload: n neurons with fan-in ``fan_in`` from uniform presynaptic
     neurons (no self-loops), weight magnitudes U[0, 0.5); 20% of the synapses are inhibitory (GABA_A
     or GABA_B, whose negative receptor sign negates the current, as in the other workloads), the rest
     AMPA or NMDA; Basal/Apical compartments, delays U{1..delay_max}, codes {0}, STDP off. Drive: every
     neuron receives an external inject of weight ``w_in`` with probability ``p_in`` per tick (soma,
-    AMPA). Connectome order is target-major."""
+    AMPA). Connectome order is target-major. Pinned regime: threshold 40 and inject weight 40 make
+    the injects (1% per neuron per tick) the firing events and keep the recurrent input (fan-in 500,
+    about 16 on average with NMDA's 100-tick time constant) subthreshold, so every neuron resets
+    about five times in 500 ticks; threshold 10 runs away within 60 ticks at 4,000 neurons."""
     e = n * fan_in
     tgt = np.repeat(np.arange(n, dtype=np.int64), fan_in)
     ctr = np.arange(e, dtype=np.uint64)

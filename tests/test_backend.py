@@ -80,3 +80,17 @@ def test_reference_learning_consistency_experiment_passes():
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
     assert "cuda_learning_consistency_experiment=passed" in r.stdout
+
+
+@pytest.mark.gpu
+@needs_check
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_inference_session_run_batch_matches_reference_cpu(seed, tmp_path):
+    """CudaInferenceSession::run_batch on the batched form: the reference's session tests restated,
+    then 5 samples drawn from a feature network's injects, each against a fresh reference CPU run
+    (stateless) and against one CPU propagator with a continuing clock (stateful)."""
+    print(f"session feature{seed}", flush=True)
+    path = tmp_path / "net.bin"
+    oracle.write_net(path, W.feature_net(seed))
+    r = _run(["session", str(path), "80", "5"])
+    assert r.returncode == 0, r.stderr
