@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--neurons", type=int, default=1_000_000)
     ap.add_argument("--stdp", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="plain form: tune the launch schedule on the warm-up ticks with trials of this many ticks")
+    ap.add_argument("--schedule", default="", help="plain form: partition_shift,scatter_workgroups,graph_ticks")
     ap.add_argument("--cpu-neurons", type=int, default=20_000)
     ap.add_argument("--cpu-ticks", type=int, default=200)
     ap.add_argument("--form", choices=("plain", "batched", "adjoint"), default="plain",
@@ -289,7 +292,15 @@ def main():
     with sa.Network(spec=spec, max_events_per_tick=1 << 27, fire_log_capacity=1 << 26,
                     raster_capacity=1 << 20, track_pre_count=False) as dev:
         dev.run_commands(injects)
+        if args.schedule:
+            ps, wg, gt = (int(x) for x in args.schedule.split(","))
+            dev.set_schedule(partition_shift=ps, scatter_workgroups=wg, graph_ticks=gt)
+        if args.autotune:
+            dev.run_until(min(99, args.warmup - 1))  # past the ignition ramp first
+            dev.autotune(args.autotune)
         dev.run_until(args.warmup - 1)
+        if args.autotune:
+            print(dev.tune_log(), file=sys.stderr, flush=True)
         # timed region: one synchronous call covering K ticks
         d0 = dev.delivered
         t0 = time.perf_counter()
@@ -306,6 +317,7 @@ def main():
         kt = dev.kernel_times()
         dpt2, _ = dev.tick_counts(args.warmup + args.steps, args.steps)
         E = dev.e
+        schedule = dev.schedule()
     scat_ms, scat_n = kt["scatter"]
     scat_avg_s = scat_ms / max(scat_n, 1) / 1e3
     ev_per_launch = float(dpt2.mean())
@@ -360,8 +372,21 @@ def main():
             "bytes_per_event": STEP_BYTES_PER_EVENT,
         },
         "kernel_ms_per_tick": {k: v[0] / max(v[1], 1) for k, v in kt.items() if v[1]},
+        "schedule": schedule,
         "ticks_per_s": args.steps / elapsed,
     }
+    # the other two forms are measured by their own invocations (bench.py --form batched / adjoint,
+    # a few minutes each); their last committed lines are attached for reference
+    forms = {}
+    for form, fname in (("batched", "batched_config3.json"), ("adjoint", "adjoint_config4.json")):
+        f = ROOT / "profiles" / fname
+        if f.exists():
+            d = json.loads(f.read_text())
+            forms[form] = {k: d.get(k) for k in ("metric", "value", "unit", "ms_per_step", "speedup_vs_loop",
+                                                 "events_per_s") if k in d}
+            forms[form]["source"] = f"profiles/{fname} (python bench.py --form {form})"
+    if forms:
+        line["forms"] = forms
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(line), flush=True)

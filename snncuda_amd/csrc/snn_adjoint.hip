@@ -72,7 +72,8 @@ int aguarded(F&& f) {
 
 constexpr uint32_t kSeg = 64;  // checkpoint interval (ticks)
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;      // reduction block
+constexpr int kNeuronBlock = 64;  // one wave per workgroup: 10,000 neurons spread over 157 CUs, not 40
 
 // One event: the synapse (connectome index) or, for an external inject, the weight's bits; tick;
 // meta = compartment | receptor << 2 | external << 4 | reset << 5.
@@ -135,7 +136,7 @@ __device__ __forceinline__ void adj_tick(const AdjArgs& a, const AdjParams& P, u
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_adj_forward(AdjArgs a, AdjParams P) {
+__global__ void __launch_bounds__(kNeuronBlock) k_adj_forward(AdjArgs a, AdjParams P) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -156,7 +157,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_forward(AdjArgs a, AdjParams P) 
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_adj_backward(AdjArgs a, AdjParams P) {
+__global__ void __launch_bounds__(kNeuronBlock) k_adj_backward(AdjArgs a, AdjParams P) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     float lam[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -527,8 +528,8 @@ int snn_adjoint_forward(snn_adjoint* adj, const float* d_weight, const float* pa
         AdjArgs a = adj->a;
         a.w = d_weight;
         a.trace = d_trace;
-        const dim3 grid((adj->n + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_adj_forward, grid, dim3(kBlock), 0, adj->stream, a, P);
+        const dim3 grid((adj->n + kNeuronBlock - 1) / kNeuronBlock);
+        hipLaunchKernelGGL(k_adj_forward, grid, dim3(kNeuronBlock), 0, adj->stream, a, P);
         acheck(hipGetLastError(), "k_adj_forward");
         acheck(hipStreamSynchronize(adj->stream), "adjoint forward");
         adj->last = P;
@@ -547,8 +548,8 @@ int snn_adjoint_backward(snn_adjoint* adj, const float* d_grad_trace, float* d_g
         a.gtrace = d_grad_trace;
         a.gw = d_grad_weight;
         if (adj->e) acheck(hipMemsetAsync(d_grad_weight, 0, adj->e * 4, adj->stream), "zero gradients");
-        const dim3 grid((adj->n + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_adj_backward, grid, dim3(kBlock), 0, adj->stream, a, adj->last);
+        const dim3 grid((adj->n + kNeuronBlock - 1) / kNeuronBlock);
+        hipLaunchKernelGGL(k_adj_backward, grid, dim3(kNeuronBlock), 0, adj->stream, a, adj->last);
         acheck(hipGetLastError(), "k_adj_backward");
         double* out = dalloc<double>(adj->own, 8);
         hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(kBlock), 0, adj->stream, (const double*)a.part, adj->n, out);

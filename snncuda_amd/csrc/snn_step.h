@@ -50,6 +50,13 @@ struct Entry16 {
     uint32_t slot;         // synapse slot (kNone: external)
 };
 
+// Bucket record: an event and its target's index within the target partition.
+template <typename E>
+struct Rec {
+    E e;
+    uint32_t tl;
+};
+
 struct Group {             // a run of the tick's due list
     uint32_t first;        // first segment index
     uint32_t nseg;
@@ -115,8 +122,14 @@ struct Step {
     // target partitions: partition p holds neurons [p << pshift, (p+1) << pshift)
     uint32_t pshift, nparts, nwg;
     uint32_t* hist;              // [nparts][nwg] per-workgroup event counts, then their bucket bases
-    void* prec;                  // partition buckets: Entry8 / Entry16 records, positions [0, D)
-    uint16_t* ptl;               // target index within its partition
+    // Code patterns kept per source neuron (code_lazy): with one code offset per synapse and the
+    // same window / similarity / max_refs on every target, a synapse's code pattern is a function of
+    // its source's fire ticks shifted by the synapse's lag, so one record per source, advanced when a
+    // fire has reached all of its synapses (lmax ticks later), gives every synapse's record.
+    uint32_t code_lazy;
+    CodeRec2* src_code;          // per neuron: its fires up to tick T - lmax applied, in fire time
+    void* prec;                  // partition buckets: Rec<Entry8> / Rec<Entry16>, positions [0, D)
+    uint32_t chunk;              // scatter write-combining chunk (records per partition buffer; 0: none)
     void* stage;                 // fold scratch for partitions larger than the LDS stage
     uint32_t* fire_bits;
     uint32_t* fire_neuron;
@@ -247,7 +260,7 @@ SNN_HD uint32_t make_event(const Step& S, uint64_t v0, uint32_t g0, uint32_t mul
         e.slot = slot;
     }
     if (CODE_HERE) {
-        if (mult) {
+        if (mult && !S.code_lazy) {
             uint32_t window = S.tp_window, max_refs = S.tp_max_refs;
             float sim = S.tp_sim;
             if (!S.tp_uniform) {
@@ -258,8 +271,8 @@ SNN_HD uint32_t make_event(const Step& S, uint64_t v0, uint32_t g0, uint32_t mul
             }
             if (CODE2) err |= code_observe2(S.code2[slot], T, mult, window, sim, max_refs);
             else err |= code_observeN(S.codeN[slot], T, mult, code_expected(S, slot), window, sim, max_refs);
-            if (S.pre_count) S.pre_count[slot] += mult;
         }
+        if (mult && S.pre_count) S.pre_count[slot] += mult;
     }
     return err;
 }

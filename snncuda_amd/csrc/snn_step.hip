@@ -306,27 +306,100 @@ __global__ void __launch_bounds__(kPBlock) k_pcount(Step S) {
     for (uint32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) S.hist[(uint64_t)p * S.nwg + blockIdx.x] = h[p];
 }
 
-// Place each event in its partition's bucket at (scanned workgroup base + LDS rank). The
-// workgroup -> event assignment is the one k_pcount used.
+// The block's events in rounds of blockDim.x consecutive positions: every thread runs the same
+// number of rounds (f gets ok = false past the end), so f may contain block barriers.
+template <bool WITH_MULT, typename F>
+__device__ void for_block_rounds(const Step& S, SegCache& c, F&& f) {
+    const uint32_t nseg = S.seg_count[0];
+    const uint64_t D = S.seg_count[1];
+    const uint32_t g0 = (uint32_t)(D * blockIdx.x / gridDim.x);
+    const uint32_t g1 = (uint32_t)(D * (blockIdx.x + 1) / gridDim.x);
+    __shared__ uint32_t qstart;
+    if (threadIdx.x == 0 && g0 < g1) qstart = seg_find(S.seg_off, nseg, g0);
+    __syncthreads();
+    if (g0 >= g1) return;
+    uint32_t qa = qstart;
+    while (true) {
+        const uint32_t qb = min(nseg, qa + kSegChunk);
+        const uint32_t nq = qb - qa;
+        for (uint32_t k = threadIdx.x; k < nq; k += blockDim.x) {
+            c.off[k] = S.seg_off[qa + k];
+            c.v0[k] = S.seg_v0[qa + k];
+            if (WITH_MULT) c.mult[k] = S.seg_mult[qa + k];
+        }
+        if (threadIdx.x == 0) c.off[nq] = qb < nseg ? S.seg_off[qb] : (uint32_t)D;
+        __syncthreads();
+        const uint32_t ga = max(g0, c.off[0]);
+        const uint32_t gb = min(g1, c.off[nq]);
+        for (uint32_t gr = ga; gr < gb; gr += blockDim.x) {
+            const uint32_t g = gr + threadIdx.x;
+            const bool ok = g < gb;
+            const uint32_t k = ok ? seg_find(c.off, nq, g) : 0u;
+            f(ok, ok ? c.v0[k] : 0ull, ok ? c.off[k] : 0u, (ok && WITH_MULT) ? c.mult[k] : 0u, ok ? g - c.off[k] : 0u);
+        }
+        __syncthreads();
+        if (gb >= g1 || qb >= nseg) break;
+        qa = qb;
+    }
+}
+
+// Place each event in its partition's bucket: the workgroup's run in partition p starts at the
+// scanned base of (p, workgroup). Records are gathered per partition in LDS, `chunk` at a time, and
+// written out as one coalesced chunk (a record that finds its partition's buffer full is written
+// straight to its place), so the bucket writes leave the workgroup as whole segments instead of
+// one scattered store per event. Positions within a run are unordered; the fold orders by g.
 template <bool WIDE, bool CODE_HERE, bool CODE2>
 __global__ void __launch_bounds__(kPBlock) k_pscatter(Step S) {
-    extern __shared__ uint32_t base[];
+    typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
+    typedef Rec<E> R;
+    extern __shared__ uint32_t dyn[];
     __shared__ SegCache cache;
-    for (uint32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) base[p] = S.hist[(uint64_t)p * S.nwg + blockIdx.x];
+    const uint32_t np = S.nparts, ch = S.chunk;
+    uint32_t* base = dyn;              // next bucket position of this workgroup's run in partition p
+    uint32_t* cnt = dyn + np;          // records taken since the last flush of partition p
+    R* buf = (R*)(dyn + 2 * np);       // [np][ch]
+    for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
+        base[p] = S.hist[(uint64_t)p * S.nwg + blockIdx.x];
+        cnt[p] = 0;
+    }
     __syncthreads();
     const uint32_t T = S.ctl[0];
     const uint32_t shift = S.pshift;
     const uint32_t mask = (1u << shift) - 1;
-    typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
+    R* out = (R*)S.prec;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
     uint32_t err = 0;
-    for_block_events<CODE_HERE>(S, cache, [&](uint64_t v0, uint32_t g0, uint32_t mult, uint32_t i) {
-        uint32_t x;
-        E e;
-        err |= make_event<WIDE, CODE_HERE, CODE2>(S, v0, g0, mult, i, T, x, e);
-        const uint32_t pos = atomicAdd(&base[x >> shift], 1u);
-        ((E*)S.prec)[pos] = e;
-        S.ptl[pos] = (uint16_t)(x & mask);
+    for_block_rounds<CODE_HERE>(S, cache, [&](bool ok, uint64_t v0, uint32_t g0, uint32_t mult, uint32_t i) {
+        if (ok) {
+            uint32_t x;
+            R r;
+            err |= make_event<WIDE, CODE_HERE, CODE2>(S, v0, g0, mult, i, T, x, r.e);
+            r.tl = x & mask;
+            const uint32_t p = x >> shift;
+            const uint32_t slot = atomicAdd(&cnt[p], 1u);
+            if (slot < ch) buf[p * ch + slot] = r;
+            else out[base[p] + slot] = r;
+        }
+        if (ch == 0) return;
+        __syncthreads();
+        // write out every full buffer: one wave per partition, one lane per record
+        for (uint32_t p = wave; p < np; p += nwaves) {
+            const uint32_t c = cnt[p];
+            if (c < ch) continue;
+            if (lane < ch) out[base[p] + lane] = buf[p * ch + lane];
+            if (lane == 0) {
+                base[p] += c;
+                cnt[p] = 0;
+            }
+        }
+        __syncthreads();
     });
+    // the partial buffers
+    for (uint32_t p = wave; p < np; p += nwaves) {
+        const uint32_t c = min(cnt[p], ch);
+        if (lane < c) out[base[p] + lane] = buf[p * ch + lane];
+    }
     if (err) atomicOr(&S.ctl[1], err);
 }
 
@@ -352,11 +425,10 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
     const uint32_t start = S.hist[(uint64_t)p * S.nwg];
     const uint32_t end = p + 1 < S.nparts ? S.hist[(uint64_t)(p + 1) * S.nwg] : S.seg_count[1];
     const uint32_t c = end - start;
-    const uint16_t* tl = S.ptl + start;
-    const E* src = (const E*)S.prec + start;
+    const Rec<E>* src = (const Rec<E>*)S.prec + start;
     for (uint32_t t = threadIdx.x; t < PS; t += blockDim.x) cnt[t] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) atomicAdd(&cnt[tl[i]], 1u);
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) atomicAdd(&cnt[src[i].tl], 1u);
     __syncthreads();
     uint32_t run = 0;
     for (uint32_t t0 = 0; t0 < PS; t0 += blockDim.x) {
@@ -377,10 +449,10 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
         __syncthreads();
         E* st = hi - lo <= kCap ? lds_stage : (E*)S.stage + start + lo;
         for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) {
-            const uint32_t ti = tl[i];
+            const uint32_t ti = src[i].tl;
             if (ti / kRound != r) continue;
             const uint32_t pos = atomicAdd(&fill[ti % kRound], 1u);
-            st[pos] = src[i];
+            st[pos] = src[i].e;
         }
         __syncthreads();
         const uint32_t x = (p << S.pshift) + t;
@@ -397,8 +469,10 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
     const uint32_t nwords = (S.seg_count[1] + 31u) >> 5;
     const uint64_t head = S.totals[2];
     const uint32_t R = S.lmax + 1;
-    // entries of ticks older than T+1-lmax may be overwritten
-    const uint64_t oldest = (S.lmax == 1) ? head : (T + 1 >= S.lmax ? S.tick_fstart[(T + 1 - S.lmax) % R] : 0);
+    // entries of ticks older than T+1-lmax may be overwritten (older than T-lmax with lazy code
+    // patterns, whose fires are applied below)
+    const uint32_t keep = S.code_lazy ? S.lmax : S.lmax - 1;  // ticks kept besides T
+    const uint64_t oldest = keep == 0 ? head : (T >= keep ? S.tick_fstart[(T - keep) % R] : 0);
     const bool fits = head + F - oldest <= S.fire_cap;
     const uint64_t rfill = S.totals[3];
     // the raster is a recording aid: when it is full, recording stops and the run goes on
@@ -423,6 +497,21 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
             }
             ++pos;
         }
+    }
+    if (S.code_lazy && T >= S.lmax) {
+        // the fires of T - lmax have reached every synapse: apply them to their sources' records
+        // (one entry per source and tick carries the multiplicity)
+        const uint32_t te = T - S.lmax;
+        const uint32_t so = te % R;
+        const uint32_t Fo = S.tick_fcount[so];
+        const uint64_t st = S.tick_fstart[so];
+        uint32_t err = 0;
+        for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < Fo; f += gridDim.x * blockDim.x) {
+            const uint64_t idx = (st + f) % S.fire_cap;
+            const uint32_t m = S.firemult[idx];
+            if (m) err |= code_observe2(S.src_code[S.firelog[idx]], te, m, S.tp_window, S.tp_sim, S.tp_max_refs);
+        }
+        if (err) atomicOr(&S.ctl[1], err);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (!fits) atomicOr(&S.ctl[1], (uint32_t)kErrFireLog);
@@ -513,6 +602,59 @@ __global__ void k_gather_ticks(const uint32_t* slot_of, const uint32_t* tpos, co
         }
         last_pre[j] = pre == kNone ? -1 : (int64_t)pre;
         last_post[j] = post == kNone ? -1 : (int64_t)post;
+    }
+}
+
+// Lazy code patterns at readback. The fires of ticks [t_first, t_first + nk) (the last lmax ticks)
+// are not in the source records yet: win[s * lmax + k] = multiplicity of s's fires at t_first + k,
+// snap[s * lmax + k] = s's record with those fires applied up to tick t_first + k.
+__global__ void k_code_window(Step S, uint32_t t_first, uint32_t* win) {
+    const uint32_t k = blockIdx.y, te = t_first + k, slot = te % (S.lmax + 1);
+    const uint32_t F = S.tick_fcount[slot];
+    const uint64_t st = S.tick_fstart[slot];
+    for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < F; f += gridDim.x * blockDim.x) {
+        const uint64_t idx = (st + f) % S.fire_cap;
+        const uint32_t m = S.firemult[idx];
+        if (m) win[(uint64_t)S.firelog[idx] * S.lmax + k] = m;
+    }
+}
+
+__global__ void k_code_snap(Step S, uint32_t t_first, uint32_t nk, const uint32_t* win, CodeRec2* snap) {
+    for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < S.n; s += (uint64_t)gridDim.x * blockDim.x) {
+        CodeRec2 r = S.src_code[s];
+        for (uint32_t k = 0; k < nk; ++k) {
+            const uint32_t m = win[s * S.lmax + k];
+            if (m) (void)code_observe2(r, t_first + k, m, S.tp_window, S.tp_sim, S.tp_max_refs);
+            snap[s * S.lmax + k] = r;
+        }
+    }
+}
+
+// Every synapse slot's record: its source's record after the fires of ticks <= T_end - lag, moved
+// from fire time to arrival time (window start + lag).
+__global__ void k_code_materialize(Step S, uint32_t T_end, uint32_t t_first, const CodeRec2* snap, uint64_t e,
+                                   CodeRec2* out) {
+    const uint32_t W = S.lmax + 1;
+    for (uint64_t slot = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; slot < e; slot += (uint64_t)gridDim.x * blockDim.x) {
+        // source: last s with row[s] <= slot (one code per synapse: rows index slots)
+        uint32_t lo = 0, hi = S.n;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (S.row[mid] <= slot) lo = mid; else hi = mid;
+        }
+        const uint32_t s = lo;
+        const uint32_t off = (uint32_t)(slot - S.row[s]);
+        // lag: last L with grp[s * W + L - 1] <= off
+        uint32_t a = 1, b = S.lmax;
+        while (a < b) {
+            const uint32_t mid = (a + b + 1) >> 1;
+            if (S.grp[(uint64_t)s * W + mid - 1] <= off) a = mid; else b = mid - 1;
+        }
+        const uint32_t L = a;
+        CodeRec2 r = S.src_code[s];
+        if (T_end >= L && T_end - L >= t_first) r = snap[(uint64_t)s * S.lmax + (T_end - L - t_first)];
+        if ((r.w1 >> 28) & 3u) r.start += L;
+        out[slot] = r;
     }
 }
 
@@ -886,16 +1028,61 @@ void launch_scan(snn_net* net, Src src, uint32_t* out, uint32_t* total, uint64_t
     hipLaunchKernelGGL(k_scan_down<Src>, dim3(kScanBlocks), dim3(kBlock), 0, net->stream, src, net->partials, out);
 }
 
+// LDS of the scatter besides its segment cache: base and count per partition plus the
+// write-combining buffers
+constexpr uint32_t kScatterDynLds = 120 * 1024;
+
 template <bool WIDE, bool CODE_HERE, bool CODE2>
 void launch_scatter(snn_net* net) {
-    hipLaunchKernelGGL((k_pscatter<WIDE, CODE_HERE, CODE2>), dim3(net->S.nwg), dim3(kPBlock), net->S.nparts * 4,
-                       net->stream, net->S);
+    typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
+    Step& S = net->S;
+    const uint32_t want = WIDE ? 8u : 16u;  // 160 / 192 bytes per flush
+    const uint64_t room = kScatterDynLds > 8ull * S.nparts ? kScatterDynLds - 8ull * S.nparts : 0;
+    uint32_t ch = (uint32_t)std::min<uint64_t>(want, room / ((uint64_t)S.nparts * sizeof(Rec<E>)));
+    if (ch < 4) ch = 0;
+    S.chunk = ch;
+    const size_t lds = 8ull * S.nparts + (size_t)S.nparts * ch * sizeof(Rec<E>);
+    hipLaunchKernelGGL((k_pscatter<WIDE, CODE_HERE, CODE2>), dim3(S.nwg), dim3(kPBlock), lds, net->stream, S);
 }
 
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
 void launch_fold(snn_net* net) {
     hipLaunchKernelGGL((k_pfold<WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kPBlock),
                        (2u * (1u << net->S.pshift) + 1) * 4, net->stream, net->S);
+}
+
+// The per-slot compact code records: the network's own, or with per-source code patterns a
+// temporary copy materialized from the source records and the fire ring's last lmax ticks.
+const CodeRec2* code2_records(snn_net* net, std::vector<DevBuf>& tmp) {
+    Step& S = net->S;
+    if (!S.code_lazy) return S.code2;
+    const uint64_t nt = net->next_tick;
+    const uint32_t T_end = nt ? (uint32_t)(nt - 1) : 0u;
+    const uint32_t t_first = nt == 0 ? 1u : (T_end >= S.lmax ? T_end - S.lmax + 1 : 0u);
+    const uint32_t nk = nt == 0 ? 0u : T_end - t_first + 1;
+    uint32_t* win = dev_alloc<uint32_t>(tmp, (uint64_t)S.n * S.lmax);
+    CodeRec2* snap = dev_alloc<CodeRec2>(tmp, (uint64_t)S.n * S.lmax, false);
+    if (nk) hipLaunchKernelGGL(k_code_window, dim3(64, nk), dim3(kBlock), 0, net->stream, S, t_first, win);
+    hipLaunchKernelGGL(k_code_snap, dim3(1024), dim3(kBlock), 0, net->stream, S, t_first, nk, (const uint32_t*)win, snap);
+    CodeRec2* out = dev_alloc<CodeRec2>(tmp, net->e, false);
+    hipLaunchKernelGGL(k_code_materialize, dim3(4096), dim3(kBlock), 0, net->stream, S, T_end, t_first,
+                       (const CodeRec2*)snap, net->e, out);
+    check(hipGetLastError(), "code records");
+    return out;
+}
+
+// once per process: let the scatter kernels take more than the default dynamic LDS
+void allow_scatter_lds() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const int bytes = (int)kScatterDynLds;
+        (void)hipFuncSetAttribute((const void*)k_pscatter<false, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_pscatter<false, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_pscatter<true, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_pscatter<true, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_pscatter<true, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_pscatter<true, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    });
 }
 
 void dispatch_scatter(snn_net* net) {
@@ -1084,9 +1271,11 @@ void run_ticks(snn_net* net, uint32_t t0, uint64_t k, Timer& tm, TickGraph& g) {
 
 // Online autotuning on the run's own ticks. Coordinate search, one schedule field at a time (graph
 // length, then scatter workgroups, then partition size) around the best so far; every candidate
-// runs `trial` ticks in each of two interleaved rounds, timed with HIP events on the stream, and
-// the lowest time per tick wins. Returns the ticks consumed (0 when the run is too short, in which
-// case tuning stays pending for a later call).
+// runs `trial` ticks in each of two interleaved rounds, timed with HIP events on the stream. The
+// score is the time per delivered event when the trial delivered at least 1e5 events per tick
+// (throughput-bound: activity drifts between trials), else the time per tick (launch-bound); the
+// lowest score wins. Returns the ticks consumed (0 when the run is too short, in which case tuning
+// stays pending for a later call).
 uint64_t autotune(snn_net* net, uint32_t t0, uint64_t nt, Timer& tm, TickGraph& g) {
     const uint32_t trial = net->tune_trial;
     auto graph_cands = [&](Schedule b) {
@@ -1122,6 +1311,7 @@ uint64_t autotune(snn_net* net, uint32_t t0, uint64_t nt, Timer& tm, TickGraph& 
     check(hipEventCreate(&a), "hipEventCreate");
     check(hipEventCreate(&b), "hipEventCreate");
     uint64_t used = 0;
+    int per_event = -1;
     Schedule best = start;
     std::string log;
     try {
@@ -1132,13 +1322,19 @@ uint64_t autotune(snn_net* net, uint32_t t0, uint64_t nt, Timer& tm, TickGraph& 
                 for (size_t c = 0; c < cands.size(); ++c) {
                     apply_schedule(net, cands[c]);
                     if (cands[c].graph && trial >= cands[c].graph) ensure_graph(net, t0, g);
+                    uint64_t d0 = 0, d1 = 0;
+                    check(hipMemcpy(&d0, net->S.totals, 8, hipMemcpyDeviceToHost), "read totals");
                     check(hipEventRecord(a, net->stream), "hipEventRecord");
                     run_ticks(net, t0, trial, tm, g);
                     check(hipEventRecord(b, net->stream), "hipEventRecord");
                     check(hipEventSynchronize(b), "hipEventSynchronize");
                     float ms = 0;
                     check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
-                    t[c] = std::min(t[c], ms / trial);
+                    check(hipMemcpy(&d1, net->S.totals, 8, hipMemcpyDeviceToHost), "read totals");
+                    const double ev = (double)(d1 - d0);
+                    if (per_event < 0) per_event = ev >= 1e5 * trial ? 1 : 0;  // decided by the first trial
+                    const float score = per_event ? (float)(ms * 1e6 / std::max(ev, 1.0)) : ms / trial;
+                    t[c] = std::min(t[c], score);
                     used += trial;
                 }
             size_t k = 0;
@@ -1147,7 +1343,7 @@ uint64_t autotune(snn_net* net, uint32_t t0, uint64_t nt, Timer& tm, TickGraph& 
             best = cands[k];
             for (size_t c = 0; c < cands.size(); ++c)
                 log += "shift=" + std::to_string(cands[c].pshift) + " wgs=" + std::to_string(cands[c].nwg) +
-                       " graph=" + std::to_string(cands[c].graph) + " ms/tick=" + std::to_string(t[c]) +
+                       " graph=" + std::to_string(cands[c].graph) + " score=" + std::to_string(t[c]) +
                        (c == k ? " *" : "") + "\n";
         }
     } catch (...) {
@@ -1212,8 +1408,6 @@ void open_net(snn_net* net, const snn_options& o, uint32_t n, uint64_t e, uint32
     Step& S = net->S;
     S.n = n;
     S.lmax = lmax;
-    if (code_compact) S.code2 = dev_alloc<CodeRec2>(own, e);
-    else S.codeN = dev_alloc<CodeRecN>(own, e);
     S.pre_count = o.track_pre_count ? dev_alloc<uint32_t>(own, e) : nullptr;
     S.dyn = dev_alloc<NeuronDyn>(own, n);
     S.th = dev_alloc<TemporalHdr>(own, n);
@@ -1243,6 +1437,13 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     S.tp_window = params[0].window;
     S.tp_max_refs = params[0].max_refs;
     S.tp_sim = params[0].sim_thr;
+    // code pattern records: per source when every synapse has one code offset and every target the
+    // same temporal parameters (SNNCUDA_CODE_PER_SYNAPSE=1 keeps them per synapse), else per synapse
+    const char* eager = std::getenv("SNNCUDA_CODE_PER_SYNAPSE");
+    S.code_lazy = (net->code_here && net->code2 && S.tp_uniform && !(eager && std::atoi(eager))) ? 1u : 0u;
+    if (S.code_lazy) S.src_code = dev_alloc<CodeRec2>(own, n);
+    else if (net->code2) S.code2 = dev_alloc<CodeRec2>(own, net->e);
+    else S.codeN = dev_alloc<CodeRecN>(own, net->e);
     const uint64_t fire_cap = o.fire_log_capacity ? o.fire_log_capacity : std::max<uint64_t>(1u << 16, 8ull * n);
     uint64_t ev_cap = o.max_events_per_tick ? o.max_events_per_tick
                                             : std::max<uint64_t>(1u << 20, std::min<uint64_t>(2 * nve, 1ull << 26));
@@ -1282,8 +1483,7 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     net->max_nwg = std::max<uint32_t>(S.nwg, 4 * net->ncu);
     net->hist_cap_parts = (uint32_t)(((uint64_t)n + 1023) >> 10);
     S.hist = dev_alloc<uint32_t>(own, (uint64_t)net->hist_cap_parts * net->max_nwg);
-    S.prec = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
-    S.ptl = dev_alloc<uint16_t>(own, ev_cap, false);
+    S.prec = net->wide ? (void*)dev_alloc<Rec<Entry16>>(own, ev_cap, false) : (void*)dev_alloc<Rec<Entry8>>(own, ev_cap, false);
     S.stage = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
     S.fire_bits = dev_alloc<uint32_t>(own, ev_cap / 32);
     S.fire_neuron = dev_alloc<uint32_t>(own, ev_cap, false);
@@ -1299,6 +1499,7 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) cus = prop.multiProcessorCount;
     net->grid = (uint32_t)std::max(64, cus * 4);
     net->fold_grid = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock, 65535ull * 4);
+    allow_scatter_lds();
     check(hipDeviceSynchronize(), "create");
 }
 
@@ -2009,11 +2210,13 @@ int snn_net_read_synapses(snn_net* net, float* weight, uint64_t* pre_count, uint
             hipLaunchKernelGGL(k_gather_weights, g, b, 0, net->stream, net->slot_of, net->S.w, net->S.tpos, net->S.stdp, e, d);
             check(hipMemcpyAsync(weight, d, e * 4, hipMemcpyDeviceToHost, net->stream), "read weights");
         }
+        const CodeRec2* c2 =
+            net->code2 && (code_match || code_learned || last_pre || last_post) ? code2_records(net, tmp) : nullptr;
         if (code_match || code_learned) {
             uint32_t* m = dev_alloc<uint32_t>(tmp, e, false);
             uint32_t* l = dev_alloc<uint32_t>(tmp, e, false);
             if (net->code2)
-                hipLaunchKernelGGL(k_gather_codes<true>, g, b, 0, net->stream, net->slot_of, net->S.code2, net->S.codeN, e, m, l);
+                hipLaunchKernelGGL(k_gather_codes<true>, g, b, 0, net->stream, net->slot_of, c2, net->S.codeN, e, m, l);
             else
                 hipLaunchKernelGGL(k_gather_codes<false>, g, b, 0, net->stream, net->slot_of, net->S.code2, net->S.codeN, e, m, l);
             std::vector<uint32_t> hm(e), hl(e);
@@ -2036,7 +2239,7 @@ int snn_net_read_synapses(snn_net* net, float* weight, uint64_t* pre_count, uint
         if (last_pre || last_post) {
             int64_t* dpre = dev_alloc<int64_t>(tmp, e, false);
             int64_t* dpost = dev_alloc<int64_t>(tmp, e, false);
-            hipLaunchKernelGGL(k_gather_ticks, g, b, 0, net->stream, net->slot_of, net->S.tpos, net->S.stdp, net->S.code2,
+            hipLaunchKernelGGL(k_gather_ticks, g, b, 0, net->stream, net->slot_of, net->S.tpos, net->S.stdp, c2,
                                net->S.codeN, net->S.delay, net->S.tgt, net->S.last_fired, e, dpre, dpost);
             if (last_pre) check(hipMemcpyAsync(last_pre, dpre, e * 8, hipMemcpyDeviceToHost, net->stream), "read ticks");
             if (last_post) check(hipMemcpyAsync(last_post, dpost, e * 8, hipMemcpyDeviceToHost, net->stream), "read ticks");

@@ -96,3 +96,22 @@ def test_device_generator_equals_host_build(plastic):
         assert np.array_equal(na[k], nb[k]), k
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("plastic", [False, True])
+def test_per_source_code_patterns_equal_per_synapse_records(plastic, monkeypatch):
+    """With one code offset per synapse and uniform target parameters the code patterns are kept per
+    source neuron; every synapse's readback must equal the per-synapse records (and the oracle),
+    also when the run ends with fires still in flight on their longest lags."""
+    net = W.config0(7, threshold=1.1, ticks=333, plastic=plastic)
+    ref = oracle.Oracle(net)
+    ref.run_commands(net.commands)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("SNNCUDA_CODE_PER_SYNAPSE", mode)
+        with run_device(net) as dev:
+            out[mode] = dev.synapses()
+    rs = ref.synapses()
+    for k in ("c_match", "c_learned", "last_pre", "last_post", "weight", "pre_count"):
+        assert np.array_equal(out["0"][k], out["1"][k]), k
+        assert np.array_equal(out["0"][k], rs[k].astype(out["0"][k].dtype)), k
