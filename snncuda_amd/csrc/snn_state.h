@@ -140,6 +140,35 @@ SNN_HD float cosine_runs(const Runs& l, uint32_t ln_runs, uint32_t l_len, const 
     return (float)(d / (sqrt(a) * sqrt(b)));
 }
 
+// cosine_runs with the right sequence's run `ov` counted as `ov_cnt` (a count held in a register)
+SNN_HD float cosine_runs_ov(const Runs& l, uint32_t l_len, const Runs& r, uint32_t r_len, uint32_t ov,
+                            uint32_t ov_cnt) {
+    if (l_len == 0 || r_len == 0) return 0.0f;
+    uint32_t k = l_len < r_len ? l_len : r_len;
+    uint64_t dot = 0, lnorm = 0, rnorm = 0;
+    uint32_t i = 0, j = 0, li = 0, rj = 0;
+    while (k > 0) {
+        const uint32_t lc = l.cnt[i];
+        const uint32_t rc = j == ov ? ov_cnt : r.cnt[j];
+        const uint32_t la = lc - li;
+        const uint32_t ra = rc - rj;
+        uint32_t take = la < ra ? la : ra;
+        if (take > k) take = k;
+        const uint64_t lv = l.val[i], rv = r.val[j];
+        dot += (uint64_t)take * lv * rv;
+        lnorm += (uint64_t)take * lv * lv;
+        rnorm += (uint64_t)take * rv * rv;
+        k -= take;
+        li += take;
+        rj += take;
+        if (li == lc) { ++i; li = 0; }
+        if (rj == rc) { ++j; rj = 0; }
+    }
+    if (lnorm == 0 || rnorm == 0) return 0.0f;
+    const double d = (double)dot, a = (double)lnorm, b = (double)rnorm;
+    return (float)(d / (sqrt(a) * sqrt(b)));
+}
+
 // Prefix length covered by the stored runs of a candidate.
 SNN_HD uint32_t stored_len(const Runs& c, uint32_t nruns) {
     uint32_t s = 0;
@@ -234,6 +263,113 @@ SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, RefsPtr refs, ui
         ml = refs->len[s] > ml ? refs->len[s] : ml;
     }
     h.maxlen = ml;
+    return err;
+}
+
+// The `times` observations at `tick` of update_temporal_pattern (SynapseDendriteProcessor.cpp:294-316)
+// that one neuron receives in one tick, with the same results as `times` calls of temporal_observe:
+// they share the tick, so only the first can restart the window and all append the same offset,
+// which lands in the candidate's last run. That run's count is kept in a register (written back
+// once), and once a match repeats with the candidate at least as long as every learned pattern,
+// every remaining observation of the tick is a repeated match and they are applied together.
+template <typename CandPtr, typename RefsPtr>
+SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, RefsPtr refs, uint32_t tick, uint32_t times,
+                                      const NeuronParams& p) {
+    uint32_t err = 0;
+    if (times == 0) return 0;
+    const bool reset = h.n == 0 || tick - h.start > p.window;
+    if (reset) {
+        h.start = tick;
+        h.n = 0;
+        h.nruns = 0;
+        h.trunc = 0;
+    }
+    const uint32_t v = tick - h.start;
+    // the run this tick's offsets go to (kRuns: none, the candidate is truncated)
+    uint32_t run = kRuns, cnt = 0;
+    if (!h.trunc) {
+        if (h.nruns > 0 && cand->val[h.nruns - 1] == v) {
+            run = h.nruns - 1;
+            cnt = cand->cnt[run];
+        } else if (h.nruns < kRuns) {
+            run = h.nruns;
+            cand->val[run] = v;
+            cnt = 0;
+            h.nruns += 1;
+        } else {
+            h.trunc = 1;
+        }
+    }
+    for (uint32_t t = 0; t < times; ++t) {
+        const bool last_match = (h.head_last >> 8) & 1u;
+        const bool repeat = !(reset && t == 0) && last_match && h.n >= h.maxlen;
+        if (repeat) {
+            // this and every later observation of the tick match
+            const uint32_t rest = times - t;
+            h.n += rest;
+            h.match += rest;
+            if (run < kRuns) cnt += rest;
+            break;
+        }
+        if (run < kRuns) cnt += 1;
+        h.n += 1;
+        bool match = false;
+        if (h.nlearned != 0 && h.n >= 2) {
+            const uint32_t have = h.trunc ? stored_len(*cand, h.nruns) : h.n;
+            float best = 0.0f;
+            const uint32_t head = h.head_last & 0xFFu;
+            for (uint32_t q = 0; q < h.nlearned; ++q) {
+                const uint32_t slot = (head + q) % kNeuronRefs;
+                const uint32_t rl = refs->len[slot];
+                const uint32_t need = rl < h.n ? rl : h.n;
+                if (need > have) {
+                    err |= kErrTemporal;
+                    continue;
+                }
+                const float c = cosine_runs_ov(refs->ref[slot], rl, *cand, need, run, cnt);
+                best = (best < c) ? c : best;
+            }
+            match = best >= p.sim_thr;
+        }
+        if (match) {
+            h.match += 1;
+            h.head_last |= 1u << 8;
+            continue;
+        }
+        h.head_last &= ~(1u << 8);
+        if (h.n < 2) continue;
+        // learn a copy of the candidate (FIFO eviction at max_refs)
+        if (h.trunc) {
+            err |= kErrTemporal;
+            continue;
+        }
+        if (run < kRuns) cand->cnt[run] = cnt;
+        uint32_t head = h.head_last & 0xFFu;
+        if (h.nlearned >= p.max_refs) {
+            head = (head + 1) % kNeuronRefs;
+            h.nlearned -= 1;
+        }
+        if (h.nlearned >= kNeuronRefs) {
+            err |= kErrTemporal;
+            continue;
+        }
+        const uint32_t slot = (head + h.nlearned) % kNeuronRefs;
+        for (uint32_t i = 0; i < h.nruns; ++i) {
+            refs->ref[slot].val[i] = cand->val[i];
+            refs->ref[slot].cnt[i] = cand->cnt[i];
+        }
+        refs->len[slot] = h.n;
+        refs->nruns[slot] = h.nruns;
+        h.nlearned += 1;
+        h.head_last = (h.head_last & ~0xFFu) | head;
+        uint32_t ml = 0;
+        for (uint32_t q = 0; q < h.nlearned; ++q) {
+            const uint32_t s2 = (head + q) % kNeuronRefs;
+            ml = refs->len[s2] > ml ? refs->len[s2] : ml;
+        }
+        h.maxlen = ml;
+    }
+    if (run < kRuns) cand->cnt[run] = cnt;
     return err;
 }
 

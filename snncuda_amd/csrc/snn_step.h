@@ -339,7 +339,6 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
     NeuronDyn d = S.dyn[x];
     TemporalHdr h = S.th[x];
     uint32_t err = 0;
-    bool cand_dirty = false;
     Runs* cand = &S.cand[x];
     decay_to(d, T, S.decay_tab);
     uint32_t fires = 0, first_g = 0;
@@ -378,7 +377,6 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
         d.cur[2] = comp == 2 ? d.cur[2] + applied_current : d.cur[2];
         d.cur[3] = comp == 3 ? d.cur[3] + applied_current : d.cur[3];
         d.taus = (d.taus & ~(3u << (2 * comp))) | (rec << (2 * comp));
-        err |= temporal_observe(h, cand, &S.refs[x], T, p, cand_dirty);
         d.membrane = d.cur[0] + d.cur[1] + d.cur[2] + d.cur[3];
         if (d.membrane >= p.threshold) {
             d.membrane = 0.0f;
@@ -396,6 +394,9 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
             ++fires;
         }
     }
+    // the neuron temporal pattern does not feed the membrane: its k observations of this tick are
+    // applied together
+    err |= temporal_observe_tick(h, cand, &S.refs[x], T, k, p);
     if (fires) {
         S.fire_mult[first_g] = fires;
         S.last_fired[x] = T;

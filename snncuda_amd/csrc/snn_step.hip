@@ -1463,10 +1463,11 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     S.seg_n = dev_alloc<uint32_t>(own, seg_cap, false);
     S.seg_off = dev_alloc<uint32_t>(own, seg_cap, false);
     S.seg_mult = dev_alloc<uint32_t>(own, seg_cap, false);
-    // Target partitions of 2048 neurons: few enough buckets that each scatter workgroup's runs per
-    // bucket stay long (writes coalesce in L2); folded in rounds of 1024 neurons. Networks of at
-    // most 1024 neurons use a single 1024-neuron partition.
-    uint32_t pshift = n <= 1024 ? 10 : 11;
+    // Target partitions of 2048 neurons (4096 from 512k neurons): few enough buckets that the
+    // scatter's write-combining buffers hold a full chunk for each; folded in rounds of 1024
+    // neurons. Networks of at most 1024 neurons use a single 1024-neuron partition. Measured on the
+    // 1M-neuron workload: 4096-neuron partitions and two scatter workgroups per CU are fastest.
+    uint32_t pshift = n <= 1024 ? 10 : (n < (1u << 19) ? 11 : 12);
     if (const char* e = std::getenv("SNNCUDA_PART_SHIFT")) pshift = std::max(10, std::min(12, std::atoi(e)));
     if (((uint64_t)n + (1ull << pshift) - 1) >> pshift > 65535)
         throw std::invalid_argument("too many neurons for 4096-neuron partitions on one device");
@@ -1476,7 +1477,7 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     hipDeviceProp_t pr;
     if (hipGetDeviceProperties(&pr, o.device) == hipSuccess) ncu = pr.multiProcessorCount;
     net->ncu = (uint32_t)std::max(1, ncu);
-    S.nwg = net->ncu;  // one scatter workgroup per CU (measured best with 1024-neuron partitions)
+    S.nwg = 2 * net->ncu;
     if (const char* e = std::getenv("SNNCUDA_SCATTER_WGS")) S.nwg = (uint32_t)std::max(1, std::atoi(e));
     // the histogram holds the largest schedule the autotuner may pick: 1024-neuron partitions and
     // four workgroups per CU
