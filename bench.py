@@ -12,10 +12,11 @@ A step is one tick. ``value`` is delivered synaptic events per second over the t
 injects excluded), with the network generated on the device and resident in HBM before timing.
 
 The JSON line also carries:
-  roofline      the dominant kernel (partition scatter) — algorithmic bytes per launch / its mean
+  roofline      the dominant kernel (the partition fold) — algorithmic bytes per launch / its mean
                 duration from HIP events on the step's stream during a second pass of the same
                 length; traffic = FETCH_SIZE + WRITE_SIZE per dispatch from rocprofv3 --pmc
                 (profiles/), scaled by the copy-kernel calibration factor stated there.
+  scatter_roofline  the same for the second kernel (the partition scatter).
   step_roofline the whole tick: 50 B per delivered event (the plain-step byte model) / tick time.
   cpu_baseline  the oracle (plain-C restatement of the reference CPU step), one thread on this
                 host, on a bounded sample of the same workload law (20,000 neurons).
@@ -24,6 +25,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import pathlib
 import sys
@@ -37,8 +39,14 @@ HBM_PEAK_GBS = 8000.0
 #   static edge fields 10 B (target, weight, delay, compartment/receptor) + event queue 8 B
 #   + synapse code-pattern record read and write 32 B
 STEP_BYTES_PER_EVENT = 50.0
-# the partition-scatter kernel performs the static edge reads and the code-pattern RMW
-SCATTER_BYTES_PER_EVENT = 10.0 + 32.0
+# partition scatter: reads the edge's target u32, weight f32 and compartment/receptor u8, writes the
+# bucket record (8 B event + 2 B target index); code patterns are advanced per source neuron
+SCATTER_BYTES_PER_EVENT = 9.0 + 10.0
+# partition fold: reads the bucket record (10 B) per event, and per touched neuron its parameters
+# (16 B), dynamic state (32 B) and temporal header (32 B), writing back state and header, and the
+# candidate's last run (4 B read, 4 B written)
+FOLD_BYTES_PER_EVENT = 10.0
+FOLD_BYTES_PER_NEURON = 16.0 + 2 * 32.0 + 2 * 32.0 + 8.0
 # STDP on adds 12 B per event (pre-side record) and 20 B per (fire, incoming plastic synapse)
 STDP_EVENT_BYTES = 12.0
 STDP_POST_BYTES = 20.0
@@ -318,18 +326,36 @@ def main():
         dpt2, _ = dev.tick_counts(args.warmup + args.steps, args.steps)
         E = dev.e
         schedule = dev.schedule()
-    scat_ms, scat_n = kt["scatter"]
-    scat_avg_s = scat_ms / max(scat_n, 1) / 1e3
     ev_per_launch = float(dpt2.mean())
-    achieved = SCATTER_BYTES_PER_EVENT * ev_per_launch / scat_avg_s / 1e9
-    traffic = None
-    tr_file = ROOT / "profiles" / "scatter_traffic.json"
-    if tr_file.exists():
-        tr = json.loads(tr_file.read_text())
-        if tr.get("stdp", False) == args.stdp:
-            # counters scaled by the copy-kernel calibration (profiles/copy_calibration.txt), per
-            # delivered event, times this run's events per launch
-            traffic = tr["bytes_per_event"] * ev_per_launch
+    # synaptic targets are uniform over the network's neurons (the input neurons only receive
+    # injects): the expected number touched by E events
+    touched = args.neurons * (1.0 - math.exp(-ev_per_launch / args.neurons))
+
+    def traffic_of(name):
+        # counters scaled by the copy-kernel calibration (profiles/copy_calibration.txt), per
+        # delivered event, times this run's events per launch
+        f = ROOT / "profiles" / name
+        if not f.exists():
+            return None
+        tr = json.loads(f.read_text())
+        return tr["bytes_per_event"] * ev_per_launch if tr.get("stdp", False) == args.stdp else None
+
+    def kernel_roofline(key, kernel, bytes_per_launch, traffic_file):
+        ms, n = kt[key]
+        avg_s = ms / max(n, 1) / 1e3
+        achieved = bytes_per_launch / avg_s / 1e9
+        return {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(traffic_file),
+                "bytes_per_launch": bytes_per_launch, "mean_launch_ms": avg_s * 1e3}
+
+    fold_rl = kernel_roofline("fold", "k_pfold (partition fold)",
+                              FOLD_BYTES_PER_EVENT * ev_per_launch + FOLD_BYTES_PER_NEURON * touched,
+                              "fold_traffic.json")
+    fold_rl["bytes_model"] = {"per_event": FOLD_BYTES_PER_EVENT, "per_touched_neuron": FOLD_BYTES_PER_NEURON,
+                              "touched_neurons": touched}
+    scat_rl = kernel_roofline("scatter", "k_pscatter (partition scatter)", SCATTER_BYTES_PER_EVENT * ev_per_launch,
+                              "scatter_traffic.json")
+    scat_rl["bytes_per_event"] = SCATTER_BYTES_PER_EVENT
     ms_per_step = elapsed / args.steps * 1e3
     step_gbs = STEP_BYTES_PER_EVENT * events / elapsed / 1e9
     line = {
@@ -356,17 +382,8 @@ def main():
             "threshold": 2.5,
             "parallelism": "1 GPU",
         },
-        "roofline": {
-            "kernel": "k_pscatter (partition scatter)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "bytes_per_event": SCATTER_BYTES_PER_EVENT,
-            "mean_launch_ms": scat_avg_s * 1e3,
-        },
+        "roofline": fold_rl,
+        "scatter_roofline": scat_rl,
         "step_roofline": {
             "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
             "bytes_per_event": STEP_BYTES_PER_EVENT,

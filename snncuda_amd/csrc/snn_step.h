@@ -50,13 +50,6 @@ struct Entry16 {
     uint32_t slot;         // synapse slot (kNone: external)
 };
 
-// Bucket record: an event and its target's index within the target partition.
-template <typename E>
-struct Rec {
-    E e;
-    uint32_t tl;
-};
-
 struct Group {             // a run of the tick's due list
     uint32_t first;        // first segment index
     uint32_t nseg;
@@ -128,7 +121,8 @@ struct Step {
     // fire has reached all of its synapses (lmax ticks later), gives every synapse's record.
     uint32_t code_lazy;
     CodeRec2* src_code;          // per neuron: its fires up to tick T - lmax applied, in fire time
-    void* prec;                  // partition buckets: Rec<Entry8> / Rec<Entry16>, positions [0, D)
+    void* prec;                  // partition buckets: Entry8 / Entry16 events, positions [0, D)
+    uint16_t* ptl;               // each bucket event's target index within its partition
     uint32_t chunk;              // scatter write-combining chunk (records per partition buffer; 0: none)
     void* stage;                 // fold scratch for partitions larger than the LDS stage
     uint32_t* fire_bits;

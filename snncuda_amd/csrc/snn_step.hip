@@ -344,20 +344,22 @@ __device__ void for_block_rounds(const Step& S, SegCache& c, F&& f) {
 }
 
 // Place each event in its partition's bucket: the workgroup's run in partition p starts at the
-// scanned base of (p, workgroup). Records are gathered per partition in LDS, `chunk` at a time, and
-// written out as one coalesced chunk (a record that finds its partition's buffer full is written
-// straight to its place), so the bucket writes leave the workgroup as whole segments instead of
-// one scattered store per event. Positions within a run are unordered; the fold orders by g.
+// scanned base of (p, workgroup). Records (the event and its 16-bit target index within the
+// partition, in two arrays) are gathered per partition in LDS, `chunk` at a time, and written out
+// as one coalesced chunk (a record that finds its partition's buffer full is written straight to
+// its place), so the bucket writes leave the workgroup as whole segments instead of one scattered
+// store per event. Positions within a run are unordered; the fold orders by g.
 template <bool WIDE, bool CODE_HERE, bool CODE2>
-__global__ void __launch_bounds__(kPBlock) k_pscatter(Step S) {
+// at most 64 VGPRs: two 1024-thread workgroups per CU (8 waves per SIMD)
+__global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8))) k_pscatter(Step S) {
     typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
-    typedef Rec<E> R;
     extern __shared__ uint32_t dyn[];
     __shared__ SegCache cache;
     const uint32_t np = S.nparts, ch = S.chunk;
     uint32_t* base = dyn;              // next bucket position of this workgroup's run in partition p
     uint32_t* cnt = dyn + np;          // records taken since the last flush of partition p
-    R* buf = (R*)(dyn + 2 * np);       // [np][ch]
+    E* buf = (E*)(dyn + 2 * np);       // [np][ch] events
+    uint16_t* buft = (uint16_t*)(buf + (size_t)np * ch);  // [np][ch] target indices
     for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
         base[p] = S.hist[(uint64_t)p * S.nwg + blockIdx.x];
         cnt[p] = 0;
@@ -366,20 +368,26 @@ __global__ void __launch_bounds__(kPBlock) k_pscatter(Step S) {
     const uint32_t T = S.ctl[0];
     const uint32_t shift = S.pshift;
     const uint32_t mask = (1u << shift) - 1;
-    R* out = (R*)S.prec;
+    E* out = (E*)S.prec;
+    uint16_t* outt = S.ptl;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
     uint32_t err = 0;
     for_block_rounds<CODE_HERE>(S, cache, [&](bool ok, uint64_t v0, uint32_t g0, uint32_t mult, uint32_t i) {
         if (ok) {
             uint32_t x;
-            R r;
-            err |= make_event<WIDE, CODE_HERE, CODE2>(S, v0, g0, mult, i, T, x, r.e);
-            r.tl = x & mask;
+            E e;
+            err |= make_event<WIDE, CODE_HERE, CODE2>(S, v0, g0, mult, i, T, x, e);
+            const uint16_t tl = (uint16_t)(x & mask);
             const uint32_t p = x >> shift;
             const uint32_t slot = atomicAdd(&cnt[p], 1u);
-            if (slot < ch) buf[p * ch + slot] = r;
-            else out[base[p] + slot] = r;
+            if (slot < ch) {
+                buf[p * ch + slot] = e;
+                buft[p * ch + slot] = tl;
+            } else {
+                out[base[p] + slot] = e;
+                outt[base[p] + slot] = tl;
+            }
         }
         if (ch == 0) return;
         __syncthreads();
@@ -387,7 +395,10 @@ __global__ void __launch_bounds__(kPBlock) k_pscatter(Step S) {
         for (uint32_t p = wave; p < np; p += nwaves) {
             const uint32_t c = cnt[p];
             if (c < ch) continue;
-            if (lane < ch) out[base[p] + lane] = buf[p * ch + lane];
+            if (lane < ch) {
+                out[base[p] + lane] = buf[p * ch + lane];
+                outt[base[p] + lane] = buft[p * ch + lane];
+            }
             if (lane == 0) {
                 base[p] += c;
                 cnt[p] = 0;
@@ -398,18 +409,23 @@ __global__ void __launch_bounds__(kPBlock) k_pscatter(Step S) {
     // the partial buffers
     for (uint32_t p = wave; p < np; p += nwaves) {
         const uint32_t c = min(cnt[p], ch);
-        if (lane < c) out[base[p] + lane] = buf[p * ch + lane];
+        if (lane < c) {
+            out[base[p] + lane] = buf[p * ch + lane];
+            outt[base[p] + lane] = buft[p * ch + lane];
+        }
     }
     if (err) atomicOr(&S.ctl[1], err);
 }
 
-constexpr uint32_t kStageBytes = 98304;
+// LDS stage of one fold round; with the 4096-neuron partition counters (32 KB) the fold uses
+// all but 3 KB of the 160 KB of LDS a workgroup may hold
+constexpr uint32_t kStageBytes = 122880;
 constexpr uint32_t kRound = kPBlock;  // targets folded per round (one per thread)
 
 // One workgroup per target partition. The partition's targets are folded in rounds of 1024 (one
 // per thread): a counting sort by target over the whole partition gives every target's place, and
-// each round stages only its own targets' records in LDS (global scratch if they do not fit), so
-// the bucket is read once for its records and once per round for the 2-byte target indices.
+// each round stages only its own targets' events in LDS (global scratch if they do not fit), so
+// the bucket's events are read once and its 2-byte target indices once plus once per round.
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
 __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
     typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
@@ -425,10 +441,22 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
     const uint32_t start = S.hist[(uint64_t)p * S.nwg];
     const uint32_t end = p + 1 < S.nparts ? S.hist[(uint64_t)(p + 1) * S.nwg] : S.seg_count[1];
     const uint32_t c = end - start;
-    const Rec<E>* src = (const Rec<E>*)S.prec + start;
+    const E* src = (const E*)S.prec + start;
+    const uint16_t* tls = S.ptl + start;
     for (uint32_t t = threadIdx.x; t < PS; t += blockDim.x) cnt[t] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) atomicAdd(&cnt[src[i].tl], 1u);
+    // the passes over the bucket load kUnroll records per thread before using them, so that many
+    // loads are in flight per wave
+    constexpr uint32_t kUnroll = 8;
+    constexpr uint32_t kNoTarget = 0xFFFFu;
+    for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * kPBlock) {
+        uint32_t tv[kUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u) tv[u] = i0 + u * kPBlock < c ? tls[i0 + u * kPBlock] : kNoTarget;
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u)
+            if (tv[u] != kNoTarget) atomicAdd(&cnt[tv[u]], 1u);
+    }
     __syncthreads();
     uint32_t run = 0;
     for (uint32_t t0 = 0; t0 < PS; t0 += blockDim.x) {
@@ -448,11 +476,17 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
         fill[threadIdx.x] = off[t] - lo;
         __syncthreads();
         E* st = hi - lo <= kCap ? lds_stage : (E*)S.stage + start + lo;
-        for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) {
-            const uint32_t ti = src[i].tl;
-            if (ti / kRound != r) continue;
-            const uint32_t pos = atomicAdd(&fill[ti % kRound], 1u);
-            st[pos] = src[i].e;
+        for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * kPBlock) {
+            uint32_t tv[kUnroll];
+            E ev[kUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) tv[u] = i0 + u * kPBlock < c ? tls[i0 + u * kPBlock] : kNoTarget;
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u)
+                if (tv[u] / kRound == r) ev[u] = src[i0 + u * kPBlock];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u)
+                if (tv[u] / kRound == r) st[atomicAdd(&fill[tv[u] % kRound], 1u)] = ev[u];
         }
         __syncthreads();
         const uint32_t x = (p << S.pshift) + t;
@@ -1036,12 +1070,13 @@ template <bool WIDE, bool CODE_HERE, bool CODE2>
 void launch_scatter(snn_net* net) {
     typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
     Step& S = net->S;
-    const uint32_t want = WIDE ? 8u : 16u;  // 160 / 192 bytes per flush
+    const uint32_t want = WIDE ? 8u : 16u;  // 128 bytes of events per flush
     const uint64_t room = kScatterDynLds > 8ull * S.nparts ? kScatterDynLds - 8ull * S.nparts : 0;
-    uint32_t ch = (uint32_t)std::min<uint64_t>(want, room / ((uint64_t)S.nparts * sizeof(Rec<E>)));
+    constexpr uint64_t kRecBytes = sizeof(E) + sizeof(uint16_t);
+    uint32_t ch = (uint32_t)std::min<uint64_t>(want, room / ((uint64_t)S.nparts * kRecBytes));
     if (ch < 4) ch = 0;
     S.chunk = ch;
-    const size_t lds = 8ull * S.nparts + (size_t)S.nparts * ch * sizeof(Rec<E>);
+    const size_t lds = 8ull * S.nparts + (size_t)S.nparts * ch * kRecBytes;
     hipLaunchKernelGGL((k_pscatter<WIDE, CODE_HERE, CODE2>), dim3(S.nwg), dim3(kPBlock), lds, net->stream, S);
 }
 
@@ -1484,7 +1519,8 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     net->max_nwg = std::max<uint32_t>(S.nwg, 4 * net->ncu);
     net->hist_cap_parts = (uint32_t)(((uint64_t)n + 1023) >> 10);
     S.hist = dev_alloc<uint32_t>(own, (uint64_t)net->hist_cap_parts * net->max_nwg);
-    S.prec = net->wide ? (void*)dev_alloc<Rec<Entry16>>(own, ev_cap, false) : (void*)dev_alloc<Rec<Entry8>>(own, ev_cap, false);
+    S.prec = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
+    S.ptl = dev_alloc<uint16_t>(own, ev_cap, false);
     S.stage = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
     S.fire_bits = dev_alloc<uint32_t>(own, ev_cap / 32);
     S.fire_neuron = dev_alloc<uint32_t>(own, ev_cap, false);
