@@ -233,11 +233,12 @@ def adjoint_form(args):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / K
     ev = adj.n_events
-    # bytes per step: events read three times (forward, recompute, reverse) at 8 B, synapse weights
-    # gathered twice and gradients written once (4 B each per event), trace written once and read once
-    # with dL/dtrace (3 x 4 B per neuron-tick), checkpoints and segment scratch (written and read once:
-    # 2 x 20 B per neuron-tick)
-    bytes_step = ev * (3 * 8 + 3 * 4) + T * adj.n * (3 * 4 + 2 * 20)
+    # bytes per step: events (8 B) read once by the forward and twice by the backward (window
+    # recompute, weight-gradient terms), the synapse weight gathered twice (4 B), each event's
+    # gradient term written and read back through the synapse's event index (4 + 4 + 4 B); per
+    # neuron-tick the trace written, the loss gradient formed from trace and target and read by the
+    # backward (4 + 12 + 4 B); per synapse its event range read and its gradient written (8 + 4 B)
+    bytes_step = ev * (3 * 8 + 2 * 4 + 3 * 4) + T * adj.n * (4 + 12 + 4) + adj.e * (8 + 4)
     line = {
         "metric": "adjoint steps/s (10k-neuron fixed-spike adjoint, forward + backward)",
         "value": 1.0 / dt,

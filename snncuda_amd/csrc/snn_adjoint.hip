@@ -16,15 +16,21 @@
 //   each event of tick t in order: c_comp += w * gain[r] * sign[r]; r_comp = r; reset event: c = 0
 //   v(t) = sum_k alpha_k c_k(t)                       (the somatic trace; alpha = 1 is the reference
 //                                                      membrane sum, :287-292)
-// Forward: one thread per neuron walks its events tick by tick, writes v [T][N] and checkpoints
-// (c, r, event position) after every 64th tick. Backward: per 64-tick segment, from the last, the
-// thread recomputes the segment from its checkpoint into a per-neuron scratch and runs the adjoint
-// recursion back through it:
-//   lambda_k += dL/dv(t) * alpha_k; dL/dalpha_k += dL/dv(t) * c_k(t)
-//   events of tick t in reverse: reset -> lambda = 0; dL/dw_syn += lambda_comp * gain * sign
-//   dL/dtau_r += lambda_k c_k(t-1) A[r] / tau_r^2 over k with r_k(t-1) = r; lambda_k *= A[r_k(t-1)]
-// A synapse's events all belong to its target, so the thread owning a neuron is the only writer of
-// its incoming synapses' weight gradients.
+// Every step of this recursion is affine in the state, with coefficients fixed by the event lists
+// (the receptor a compartment decays with is that of its last input, known from the events alone).
+// Each neuron is one wave and each 64-tick window one pass of it, lane j holding tick t0 + j:
+//   forward: lane j composes its tick's events into c -> ea*c + eb per compartment, a wave scan of
+//   "last receptor written" gives the receptor each tick decays with, and a wave scan of the affine
+//   maps gives c after every tick of the window from the state at its start. The trace is written
+//   and the state at every window start (every 64 ticks) is the checkpoint.
+//   backward: windows from the last; the window's c(t) is recomputed from its checkpoint the same
+//   way, the adjoint lambda enters from the later window and is carried back through the window by a
+//   reverse wave scan of its (also affine) per-tick maps:
+//     lambda_k += dL/dv(t) * alpha_k; events of tick t in reverse: reset -> lambda = 0;
+//     dL/dw_syn += lambda_comp * gain * sign; dL/dalpha_k += dL/dv(t) * c_k(t);
+//     dL/dtau_r += lambda_k c_k(t-1) A[r] / tau_r^2 over k with r_k(t-1) = r; lambda_k *= A[r_k(t-1)]
+//   Each synaptic event's weight-gradient term is written at its event position; a last kernel sums
+//   each synapse's terms in event order (fixed order, no atomics).
 
 #include <hip/hip_runtime.h>
 
@@ -70,10 +76,11 @@ int aguarded(F&& f) {
     }
 }
 
-constexpr uint32_t kSeg = 64;  // checkpoint interval (ticks)
+constexpr uint32_t kSeg = 64;     // window = checkpoint interval (ticks): one lane per tick
+constexpr uint32_t kWave = 64;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr int kBlock = 256;      // reduction block
-constexpr int kNeuronBlock = 64;  // one wave per workgroup: 10,000 neurons spread over 157 CUs, not 40
+constexpr int kBlock = 256;       // reduction block
+constexpr int kWavesPerBlock = 4; // neurons per workgroup, one wave each
 
 // One event: the synapse (connectome index) or, for an external inject, the weight's bits; tick;
 // meta = compartment | receptor << 2 | external << 4 | reset << 5.
@@ -98,12 +105,13 @@ struct AdjArgs {
     const AdjEvent* ev;
     const float* w;         // synapse weights, connectome order
     float* trace;           // [T][n]
-    float* ck;              // [nseg][4][n] state after tick 64 s + 63
-    uint32_t* ckr;          // [nseg][n] receptor codes (2 bits per compartment)
-    uint64_t* ckp;          // [nseg + 1][n] event position at the start of segment s
-    float* scr;             // [64][4][n] segment recompute scratch
-    uint32_t* scr_r;        // [64][n]
+    float* ck;              // [nseg][4][n] state at the start of window s
+    uint32_t* ckr;          // [nseg][n] receptor codes (2 bits per compartment) at the start of window s
+    uint64_t* ckp;          // [nseg + 1][n] event position at the start of window s
     const float* gtrace;    // [T][n]
+    float* gev;             // per event: its weight-gradient term (synaptic events)
+    const uint64_t* sptr;   // e + 1: synapse s's events are sev[sptr[s], sptr[s+1]), in event order
+    const uint32_t* sev;
     float* gw;              // synapse weight gradients, connectome order
     double* part;           // [n][8] per-neuron dL/dalpha, dL/dtau
 };
@@ -114,128 +122,282 @@ __device__ __forceinline__ float ev_input(const AdjEvent& e, const float* w) {
     return wt * kGainSign[r];
 }
 
-// one tick of the forward dynamics for neuron i: decay, then the events of tick t from *p on
-__device__ __forceinline__ void adj_tick(const AdjArgs& a, const AdjParams& P, uint32_t t, uint64_t& p, uint64_t end,
-                                         float c[4], uint32_t& rk) {
-    if (t > 0) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) c[k] *= P.A[(rk >> (2 * k)) & 3u];
-    }
-    for (; p < end; ++p) {
-        const AdjEvent e = a.ev[p];
-        if (e.tick != t) break;
-        const uint32_t k = e.meta & 3u;
-        const uint32_t r = (e.meta >> 2) & 3u;
-        const float u = ev_input(e, a.w);
-        c[0] = k == 0 ? c[0] + u : c[0];
-        c[1] = k == 1 ? c[1] + u : c[1];
-        c[2] = k == 2 ? c[2] + u : c[2];
-        c[3] = k == 3 ? c[3] + u : c[3];
-        rk = (rk & ~(3u << (2 * k))) | (r << (2 * k));
-        if (e.meta & 32u) c[0] = c[1] = c[2] = c[3] = 0.0f;
-    }
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
-__global__ void __launch_bounds__(kNeuronBlock) k_adj_forward(AdjArgs a, AdjParams P) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    uint32_t rk = 0;
-    uint64_t p = a.ptr[i];
-    const uint64_t end = a.ptr[i + 1];
-    a.ckp[i] = p;
-    for (uint32_t t = 0; t < a.T; ++t) {
-        adj_tick(a, P, t, p, end, c, rk);
-        a.trace[(uint64_t)t * a.n + i] = P.alpha[0] * c[0] + P.alpha[1] * c[1] + P.alpha[2] * c[2] + P.alpha[3] * c[3];
-        if ((t % kSeg) == kSeg - 1 || t + 1 == a.T) {
-            const uint32_t s = t / kSeg;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) a.ck[((uint64_t)s * 4 + k) * a.n + i] = c[k];
-            a.ckr[(uint64_t)s * a.n + i] = rk;
-            a.ckp[(uint64_t)(s + 1) * a.n + i] = p;
+// bit k of wr set -> the 2-bit receptor field of compartment k
+__device__ __forceinline__ uint32_t field_mask(uint32_t wr) {
+    return ((wr & 1u) ? 3u : 0u) | ((wr & 2u) ? 12u : 0u) | ((wr & 4u) ? 48u : 0u) | ((wr & 8u) ? 192u : 0u);
+}
+
+__device__ __forceinline__ float sel4(const float v[4], uint32_t k) {
+    return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : v[3];
+}
+
+struct WaveLds {                 // one wave's LDS
+    float u[kWave];              // a batch of events: input, meta | tick offset << 8
+    uint32_t md[kWave];
+    uint32_t d[kWave];
+    uint32_t lo[kWave], hi[kWave];  // per tick offset: its events' range within the batch
+    float lam[kWave][4];         // backward: per tick, lambda seen by events after its last reset
+    uint32_t lastr[kWave];       // backward: 1 + window position of the tick's last reset (0: none)
+};
+
+// The events of tick t0 + lane as c_k -> ea_k * c_k + eb_k (decay excluded), the compartments they
+// write (wr) with their last receptors (lr), and the window position of the last reset (+1, 0: none).
+struct TickMap {
+    float ea[4], eb[4];
+    uint32_t wr, lr, lastr;
+};
+
+// Reads the window's events from position p (at most up to end) in batches of 64, one event per
+// lane; lane j applies the batch's events of tick t0 + j in order. Returns the position after the
+// window's events.
+__device__ uint64_t window_maps(const AdjArgs& a, uint64_t p, uint64_t end, uint32_t t0, uint32_t tw, uint32_t lane,
+                                WaveLds& L, TickMap& m) {
+    for (int k = 0; k < 4; ++k) {
+        m.ea[k] = 1.0f;
+        m.eb[k] = 0.0f;
+    }
+    m.wr = 0;
+    m.lr = 0;
+    m.lastr = 0;
+    const uint64_t pw = p;
+    while (true) {
+        const uint64_t q = p + lane;
+        uint32_t d = kWave;  // not in the window
+        AdjEvent e{};
+        if (q < end) {
+            e = a.ev[q];
+            d = (uint32_t)e.tick - t0;  // events before the window were consumed, so tick >= t0
+        }
+        const bool in = d < tw;
+        const uint32_t cnt = (uint32_t)__popcll(__ballot(in));  // a prefix: events are sorted by tick
+        L.d[lane] = in ? d : kWave;
+        L.lo[lane] = 0;
+        L.hi[lane] = 0;
+        if (in) {
+            L.u[lane] = ev_input(e, a.w);
+            L.md[lane] = e.meta;
+        }
+        wave_sync();
+        if (in) {
+            if (lane == 0 || L.d[lane - 1] != d) L.lo[d] = lane;
+            if (lane + 1 == kWave || L.d[lane + 1] != d) L.hi[d] = lane + 1;
+        }
+        wave_sync();
+        const uint32_t lo = L.lo[lane], hi = L.hi[lane];
+        for (uint32_t x = lo; x < hi; ++x) {
+            const uint32_t md = L.md[x];
+            const float u = L.u[x];
+            const uint32_t k = md & 3u, r = (md >> 2) & 3u;
+            m.eb[0] = k == 0 ? m.eb[0] + u : m.eb[0];
+            m.eb[1] = k == 1 ? m.eb[1] + u : m.eb[1];
+            m.eb[2] = k == 2 ? m.eb[2] + u : m.eb[2];
+            m.eb[3] = k == 3 ? m.eb[3] + u : m.eb[3];
+            m.wr |= 1u << k;
+            m.lr = (m.lr & ~(3u << (2 * k))) | (r << (2 * k));
+            if (md & 32u) {
+                for (int c = 0; c < 4; ++c) {
+                    m.ea[c] = 0.0f;
+                    m.eb[c] = 0.0f;
+                }
+                m.lastr = (uint32_t)(p + x - pw) + 1;
+            }
+        }
+        wave_sync();
+        p += cnt;
+        if (cnt < kWave) break;
+    }
+    return p;
+}
+
+// receptor codes before tick t0 + lane: r_start updated by the writes of the ticks before it
+__device__ __forceinline__ uint32_t receptors_in(const TickMap& m, uint32_t r_start, uint32_t lane) {
+    uint32_t swr = m.wr, slr = m.lr;
+    for (uint32_t off = 1; off < kWave; off <<= 1) {
+        const uint32_t owr = __shfl_up(swr, off, kWave), olr = __shfl_up(slr, off, kWave);
+        if (lane >= off) {
+            const uint32_t M = field_mask(swr);
+            slr = (olr & ~M) | (slr & M);
+            swr |= owr;
         }
     }
+    uint32_t pwr = __shfl_up(swr, 1, kWave), plr = __shfl_up(slr, 1, kWave);
+    if (lane == 0) pwr = plr = 0;
+    const uint32_t M = field_mask(pwr);
+    return (r_start & ~M) | (plr & M);
 }
 
-__global__ void __launch_bounds__(kNeuronBlock) k_adj_backward(AdjArgs a, AdjParams P) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    float lam[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+// c after tick t0 + lane, from the state c0 at the window start
+__device__ __forceinline__ void states_after(const TickMap& m, const AdjParams& P, uint32_t r_in, bool decays,
+                                             const float c0[4], uint32_t lane, float c[4]) {
+    float am[4], bm[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float D = decays ? P.A[(r_in >> (2 * k)) & 3u] : 1.0f;
+        am[k] = m.ea[k] * D;
+        bm[k] = m.eb[k];
+    }
+    for (uint32_t off = 1; off < kWave; off <<= 1) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float oa = __shfl_up(am[k], off, kWave), ob = __shfl_up(bm[k], off, kWave);
+            if (lane >= off) {
+                bm[k] = am[k] * ob + bm[k];
+                am[k] = am[k] * oa;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c[k] = am[k] * c0[k] + bm[k];
+}
+
+__global__ void __launch_bounds__(kWavesPerBlock * kWave) k_adj_forward(AdjArgs a, AdjParams P) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wv = threadIdx.x / kWave;
+    const uint32_t i = blockIdx.x * kWavesPerBlock + wv;
+    if (i >= a.n) return;  // whole wave; no block barriers below
+    WaveLds& L = lds[wv];
+    float c0[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t r0 = 0;
+    uint64_t p = a.ptr[i];
+    const uint64_t end = a.ptr[i + 1];
+    for (uint32_t s = 0; s < a.nseg; ++s) {
+        const uint32_t t0 = s * kSeg, tw = min(kSeg, a.T - t0);
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) a.ck[((uint64_t)s * 4 + k) * a.n + i] = c0[k];
+            a.ckr[(uint64_t)s * a.n + i] = r0;
+            a.ckp[(uint64_t)s * a.n + i] = p;
+        }
+        TickMap m;
+        p = window_maps(a, p, end, t0, tw, lane, L, m);
+        const uint32_t r_in = receptors_in(m, r0, lane);
+        float c[4];
+        states_after(m, P, r_in, t0 + lane > 0, c0, lane, c);
+        if (lane < tw)
+            a.trace[(uint64_t)(t0 + lane) * a.n + i] =
+                P.alpha[0] * c[0] + P.alpha[1] * c[1] + P.alpha[2] * c[2] + P.alpha[3] * c[3];
+        const uint32_t r_out = (r_in & ~field_mask(m.wr)) | (m.lr & field_mask(m.wr));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c0[k] = __shfl(c[k], (int)(tw - 1), kWave);
+        r0 = __shfl(r_out, (int)(tw - 1), kWave);
+    }
+    if (lane == 0) a.ckp[(uint64_t)a.nseg * a.n + i] = p;
+}
+
+__global__ void __launch_bounds__(kWavesPerBlock * kWave) k_adj_backward(AdjArgs a, AdjParams P) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wv = threadIdx.x / kWave;
+    const uint32_t i = blockIdx.x * kWavesPerBlock + wv;
+    if (i >= a.n) return;  // whole wave; no block barriers below
+    WaveLds& L = lds[wv];
+    float carry[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // lambda entering the window's last tick
     float galpha[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     float gtau[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     for (uint32_t s = a.nseg; s-- > 0;) {
-        const uint32_t t0 = s * kSeg, t1 = min(a.T, t0 + kSeg);
-        // state before the segment
-        float c0[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        uint32_t r0 = 0;
-        if (s > 0) {
+        const uint32_t t0 = s * kSeg, tw = min(kSeg, a.T - t0);
+        const uint32_t t = t0 + lane;
+        const bool live = lane < tw;
+        float c0[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) c0[k] = a.ck[((uint64_t)(s - 1) * 4 + k) * a.n + i];
-            r0 = a.ckr[(uint64_t)(s - 1) * a.n + i];
-        }
-        // recompute the segment from its checkpoint
-        {
-            float c[4] = {c0[0], c0[1], c0[2], c0[3]};
-            uint32_t rk = r0;
-            uint64_t p = a.ckp[(uint64_t)s * a.n + i];
-            const uint64_t end = a.ckp[(uint64_t)(s + 1) * a.n + i];
-            for (uint32_t t = t0; t < t1; ++t) {
-                adj_tick(a, P, t, p, end, c, rk);
+        for (int k = 0; k < 4; ++k) c0[k] = a.ck[((uint64_t)s * 4 + k) * a.n + i];
+        const uint32_t r0 = a.ckr[(uint64_t)s * a.n + i];
+        const uint64_t pbeg = a.ckp[(uint64_t)s * a.n + i], pend = a.ckp[(uint64_t)(s + 1) * a.n + i];
+        // the window's forward states
+        TickMap m;
+        window_maps(a, pbeg, pend, t0, tw, lane, L, m);
+        const uint32_t r_in = receptors_in(m, r0, lane);
+        float c[4], cp[4];
+        states_after(m, P, r_in, t > 0, c0, lane, c);
 #pragma unroll
-                for (int k = 0; k < 4; ++k) a.scr[((uint64_t)(t - t0) * 4 + k) * a.n + i] = c[k];
-                a.scr_r[(uint64_t)(t - t0) * a.n + i] = rk;
-            }
+        for (int k = 0; k < 4; ++k) {
+            cp[k] = __shfl_up(c[k], 1, kWave);
+            if (lane == 0) cp[k] = c0[k];
         }
-        // adjoint recursion back through the segment
-        uint64_t p = a.ckp[(uint64_t)(s + 1) * a.n + i];
-        const uint64_t pbeg = a.ckp[(uint64_t)s * a.n + i];
-        for (uint32_t t = t1; t-- > t0;) {
-            const float g = a.gtrace[(uint64_t)t * a.n + i];
+        // lambda maps: lambda after tick t = mm * (lambda entering tick t) + qq
+        const float g = live ? a.gtrace[(uint64_t)t * a.n + i] : 0.0f;
+        const bool reset = m.lastr != 0;
+        float mm[4], qq[4], D[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            D[k] = t > 0 ? P.A[(r_in >> (2 * k)) & 3u] : 1.0f;
+            mm[k] = !live ? 1.0f : reset ? 0.0f : D[k];
+            qq[k] = (live && !reset) ? D[k] * (g * P.alpha[k]) : 0.0f;
+        }
+        for (uint32_t off = 1; off < kWave; off <<= 1) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                galpha[k] += g * a.scr[((uint64_t)(t - t0) * 4 + k) * a.n + i];
-                lam[k] += g * P.alpha[k];
-            }
-            for (; p > pbeg; --p) {
-                const AdjEvent e = a.ev[p - 1];
-                if (e.tick != t) break;
-                if (e.meta & 32u) lam[0] = lam[1] = lam[2] = lam[3] = 0.0f;
-                if (!(e.meta & 16u)) {
-                    const uint32_t k = e.meta & 3u;
-                    const uint32_t r = (e.meta >> 2) & 3u;
-                    const float lk = k == 0 ? lam[0] : k == 1 ? lam[1] : k == 2 ? lam[2] : lam[3];
-                    a.gw[e.ref] += lk * kGainSign[r];
+                const float om = __shfl_down(mm[k], off, kWave), oq = __shfl_down(qq[k], off, kWave);
+                if (lane + off < kWave) {
+                    qq[k] = mm[k] * oq + qq[k];
+                    mm[k] = mm[k] * om;
                 }
             }
-            if (t > 0) {
-                float cp[4];
-                uint32_t rp;
-                if (t > t0) {
+        }
+        float lin[4];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) cp[k] = a.scr[((uint64_t)(t - 1 - t0) * 4 + k) * a.n + i];
-                    rp = a.scr_r[(uint64_t)(t - 1 - t0) * a.n + i];
-                } else {
+        for (int k = 0; k < 4; ++k) {
+            const float lout = mm[k] * carry[k] + qq[k];
+            lin[k] = __shfl_down(lout, 1, kWave);
+            if (lane + 1 == kWave) lin[k] = carry[k];
+            const float l0 = __shfl(lout, 0, kWave);
+            carry[k] = l0;
+        }
+        if (live) {
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) cp[k] = c0[k];
-                    rp = r0;
-                }
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const uint32_t r = (rp >> (2 * k)) & 3u;
-                    const float gt = lam[k] * cp[k] * P.dA[r];
+            for (int k = 0; k < 4; ++k) {
+                const float pre = lin[k] + g * P.alpha[k];
+                L.lam[lane][k] = pre;
+                galpha[k] += g * c[k];
+                if (t > 0) {
+                    const uint32_t r = (r_in >> (2 * k)) & 3u;
+                    const float gt = (reset ? 0.0f : pre) * cp[k] * P.dA[r];
                     gtau[0] += r == 0 ? gt : 0.0f;
                     gtau[1] += r == 1 ? gt : 0.0f;
                     gtau[2] += r == 2 ? gt : 0.0f;
                     gtau[3] += r == 3 ? gt : 0.0f;
-                    lam[k] *= P.A[r];
+                }
+            }
+            L.lastr[lane] = m.lastr;
+        }
+        wave_sync();
+        // weight-gradient terms of the window's synaptic events
+        for (uint64_t b = pbeg; b < pend; b += kWave) {
+            const uint64_t q = b + lane;
+            if (q < pend) {
+                const AdjEvent e = a.ev[q];
+                if (!(e.meta & 16u)) {
+                    const uint32_t d = (uint32_t)e.tick - t0;
+                    const uint32_t k = e.meta & 3u, r = (e.meta >> 2) & 3u;
+                    const float lam = (uint32_t)(q - pbeg) + 1 > L.lastr[d] ? sel4(L.lam[d], k) : 0.0f;
+                    a.gev[q] = lam * kGainSign[r];
                 }
             }
         }
+        wave_sync();
     }
+    // per-neuron partials: lane sums in FP64, fixed order
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        a.part[(uint64_t)i * 8 + k] = galpha[k];
-        a.part[(uint64_t)i * 8 + 4 + k] = gtau[k];
+    for (int k = 0; k < 8; ++k) {
+        double v = k < 4 ? (double)galpha[k] : (double)gtau[k - 4];
+        for (uint32_t off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+        if (lane == 0) a.part[(uint64_t)i * 8 + k] = v;
+    }
+}
+
+// each synapse's weight gradient: the sum of its events' terms in event order
+__global__ void __launch_bounds__(kBlock) k_adj_weight_grad(const uint64_t* sptr, const uint32_t* sev, const float* gev,
+                                                            uint64_t e, float* gw) {
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < e; j += (uint64_t)gridDim.x * blockDim.x) {
+        float acc = 0.0f;
+        for (uint64_t q = sptr[j]; q < sptr[j + 1]; ++q) acc += gev[sev[q]];
+        gw[j] = acc;
     }
 }
 
@@ -280,6 +442,7 @@ struct snn_adjoint {
     std::vector<std::unique_ptr<DBuf>> own;
     AdjArgs a{};
     AdjParams last{};
+    double* sums = nullptr;  // the eight parameter gradients
     bool have_forward = false;
     std::vector<uint64_t> h_ptr;
     std::vector<AdjEvent> h_ev;
@@ -479,9 +642,26 @@ int snn_adjoint_create(snn_net* run, const snn_net_desc* desc, uint64_t ticks, s
         A.ck = dalloc<float>(adj->own, (uint64_t)adj->nseg * 4 * n);
         A.ckr = dalloc<uint32_t>(adj->own, (uint64_t)adj->nseg * n);
         A.ckp = dalloc<uint64_t>(adj->own, (uint64_t)(adj->nseg + 1) * n);
-        A.scr = dalloc<float>(adj->own, (uint64_t)kSeg * 4 * n);
-        A.scr_r = dalloc<uint32_t>(adj->own, (uint64_t)kSeg * n);
         A.part = dalloc<double>(adj->own, (uint64_t)n * 8);
+        A.gev = dalloc<float>(adj->own, adj->nev);
+        // each synapse's events, in event order
+        {
+            std::vector<uint64_t> sptr(e + 1, 0);
+            for (const AdjEvent& ev : adj->h_ev)
+                if (!(ev.meta & 16u)) sptr[ev.ref + 1]++;
+            for (uint64_t j = 0; j < e; ++j) sptr[j + 1] += sptr[j];
+            std::vector<uint32_t> sev(sptr[e]);
+            std::vector<uint64_t> fill(sptr.begin(), sptr.end() - 1);
+            for (uint64_t q = 0; q < adj->nev; ++q)
+                if (!(adj->h_ev[q].meta & 16u)) sev[fill[adj->h_ev[q].ref]++] = (uint32_t)q;
+            uint64_t* dsp = dalloc<uint64_t>(adj->own, e + 1);
+            uint32_t* dse = dalloc<uint32_t>(adj->own, sev.size());
+            acheck(hipMemcpy(dsp, sptr.data(), (e + 1) * 8, hipMemcpyHostToDevice), "upload");
+            if (!sev.empty()) acheck(hipMemcpy(dse, sev.data(), sev.size() * 4, hipMemcpyHostToDevice), "upload");
+            A.sptr = dsp;
+            A.sev = dse;
+        }
+        adj->sums = dalloc<double>(adj->own, 8);
         *out = adj.release();
         return SNN_OK;
     });
@@ -528,8 +708,8 @@ int snn_adjoint_forward(snn_adjoint* adj, const float* d_weight, const float* pa
         AdjArgs a = adj->a;
         a.w = d_weight;
         a.trace = d_trace;
-        const dim3 grid((adj->n + kNeuronBlock - 1) / kNeuronBlock);
-        hipLaunchKernelGGL(k_adj_forward, grid, dim3(kNeuronBlock), 0, adj->stream, a, P);
+        const dim3 grid((adj->n + kWavesPerBlock - 1) / kWavesPerBlock);
+        hipLaunchKernelGGL(k_adj_forward, grid, dim3(kWavesPerBlock * kWave), 0, adj->stream, a, P);
         acheck(hipGetLastError(), "k_adj_forward");
         acheck(hipStreamSynchronize(adj->stream), "adjoint forward");
         adj->last = P;
@@ -547,15 +727,18 @@ int snn_adjoint_backward(snn_adjoint* adj, const float* d_grad_trace, float* d_g
         AdjArgs a = adj->a;
         a.gtrace = d_grad_trace;
         a.gw = d_grad_weight;
-        if (adj->e) acheck(hipMemsetAsync(d_grad_weight, 0, adj->e * 4, adj->stream), "zero gradients");
-        const dim3 grid((adj->n + kNeuronBlock - 1) / kNeuronBlock);
-        hipLaunchKernelGGL(k_adj_backward, grid, dim3(kNeuronBlock), 0, adj->stream, a, adj->last);
+        const dim3 grid((adj->n + kWavesPerBlock - 1) / kWavesPerBlock);
+        hipLaunchKernelGGL(k_adj_backward, grid, dim3(kWavesPerBlock * kWave), 0, adj->stream, a, adj->last);
         acheck(hipGetLastError(), "k_adj_backward");
-        double* out = dalloc<double>(adj->own, 8);
-        hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(kBlock), 0, adj->stream, (const double*)a.part, adj->n, out);
-        acheck(hipMemcpyAsync(grad_params, out, 8 * sizeof(double), hipMemcpyDeviceToHost, adj->stream), "read");
+        if (adj->e) {
+            const uint64_t nb = std::min<uint64_t>((adj->e + kBlock - 1) / kBlock, 8192);
+            hipLaunchKernelGGL(k_adj_weight_grad, dim3((uint32_t)nb), dim3(kBlock), 0, adj->stream, a.sptr, a.sev,
+                               (const float*)a.gev, adj->e, d_grad_weight);
+            acheck(hipGetLastError(), "k_adj_weight_grad");
+        }
+        hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(kBlock), 0, adj->stream, (const double*)a.part, adj->n, adj->sums);
+        acheck(hipMemcpyAsync(grad_params, adj->sums, 8 * sizeof(double), hipMemcpyDeviceToHost, adj->stream), "read");
         acheck(hipStreamSynchronize(adj->stream), "adjoint backward");
-        adj->own.pop_back();
         return SNN_OK;
     });
 }
