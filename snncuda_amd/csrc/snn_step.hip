@@ -134,37 +134,56 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(Src src, uint32_t* parti
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
-// exclusive scan of the block partials; writes the grand total to *total and, when cap is
-// exceeded, raises err_flag and zeroes *zero_on_overflow (so the tick does no further work)
-__global__ void __launch_bounds__(kBlock) k_scan_mid(uint32_t* partials, uint32_t nparts, uint32_t* total,
-                                                     uint64_t cap, uint32_t err_flag, uint32_t* err,
-                                                     uint32_t* zero_on_overflow) {
-    uint32_t base = 0;
-    for (uint32_t off = 0; off < nparts; off += blockDim.x) {
-        const uint32_t i = off + threadIdx.x;
-        const uint32_t v = i < nparts ? partials[i] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan(v, &tot);
-        if (i < nparts) partials[i] = base + ex;
-        base += tot;
+// sum over the block of two values per thread (kBlock threads)
+__device__ __forceinline__ void block_sum2(uint32_t& a, uint32_t& b) {
+    __shared__ uint32_t wa[kBlock / kWave], wb[kBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        a += __shfl_down(a, o, kWave);
+        b += __shfl_down(b, o, kWave);
     }
-    if (threadIdx.x == 0) {
-        if (total) *total = base;
-        if (cap && base > cap) {
+    if (lane == 0) {
+        wa[wid] = a;
+        wb[wid] = b;
+    }
+    __syncthreads();
+    a = b = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) {
+        a += wa[w];
+        b += wb[w];
+    }
+    __syncthreads();
+}
+
+// Down-sweep of the scan. Each block forms its base from the block partials itself (the sum of
+// those before it) and block 0 the grand total: *total, and when cap is exceeded err_flag raised
+// and *zero_on_overflow and *total zeroed (so the tick does no further work).
+template <class Src>
+__global__ void __launch_bounds__(kBlock) k_scan_down(Src src, const uint32_t* partials, uint32_t* out, uint32_t* total,
+                                                      uint64_t cap, uint32_t err_flag, uint32_t* err,
+                                                      uint32_t* zero_on_overflow) {
+    const uint32_t n = src.count();
+    uint32_t before = 0, all = 0;
+    for (uint32_t j = threadIdx.x; j < gridDim.x; j += blockDim.x) {
+        const uint32_t v = partials[j];
+        all += v;
+        before += j < blockIdx.x ? v : 0u;
+    }
+    block_sum2(before, all);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (total) *total = all;
+        if (cap && all > cap) {
             atomicOr(err, err_flag);
             if (zero_on_overflow) *zero_on_overflow = 0;
             if (total) *total = 0;
         }
     }
-}
-
-template <class Src>
-__global__ void __launch_bounds__(kBlock) k_scan_down(Src src, const uint32_t* partials, uint32_t* out) {
-    const uint32_t n = src.count();
     const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const uint32_t lo = min(n, blockIdx.x * chunk);
     const uint32_t hi = min(n, lo + chunk);
-    uint32_t base = partials[blockIdx.x];
+    uint32_t base = before;
     for (uint32_t off = lo; off < hi; off += blockDim.x) {
         const uint32_t i = off + threadIdx.x;
         const uint32_t v = i < hi ? src.load(i) : 0u;
@@ -299,16 +318,55 @@ __global__ void __launch_bounds__(kPBlock) k_pcount(Step S) {
     for (uint32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) h[p] = 0;
     __syncthreads();
     const uint32_t shift = S.pshift;
-    for_block_events<false>(S, cache, [&](uint64_t v0, uint32_t, uint32_t, uint32_t i) {
-        atomicAdd(&h[event_target(S, v0, i) >> shift], 1u);
-    });
+    // kCountUnroll events per thread at a time: their target loads are all in flight before the
+    // histogram updates
+    constexpr uint32_t kCountUnroll = 4;
+    const uint32_t nseg = S.seg_count[0];
+    const uint64_t D = S.seg_count[1];
+    const uint32_t g0 = (uint32_t)(D * blockIdx.x / gridDim.x);
+    const uint32_t g1 = (uint32_t)(D * (blockIdx.x + 1) / gridDim.x);
+    __shared__ uint32_t qstart;
+    if (threadIdx.x == 0 && g0 < g1) qstart = seg_find(S.seg_off, nseg, g0);
+    __syncthreads();
+    uint32_t qa = g0 < g1 ? qstart : nseg;
+    while (g0 < g1) {
+        const uint32_t qb = min(nseg, qa + kSegChunk);
+        const uint32_t nq = qb - qa;
+        for (uint32_t k = threadIdx.x; k < nq; k += blockDim.x) {
+            cache.off[k] = S.seg_off[qa + k];
+            cache.v0[k] = S.seg_v0[qa + k];
+        }
+        if (threadIdx.x == 0) cache.off[nq] = qb < nseg ? S.seg_off[qb] : (uint32_t)D;
+        __syncthreads();
+        const uint32_t ga = max(g0, cache.off[0]);
+        const uint32_t gb = min(g1, cache.off[nq]);
+        for (uint32_t g = ga + threadIdx.x; g < gb; g += kCountUnroll * blockDim.x) {
+            uint32_t x[kCountUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kCountUnroll; ++u) {
+                const uint32_t gg = g + u * blockDim.x;
+                x[u] = 0xFFFFFFFFu;
+                if (gg < gb) {
+                    const uint32_t k = seg_find(cache.off, nq, gg);
+                    x[u] = event_target(S, cache.v0[k], gg - cache.off[k]);
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kCountUnroll; ++u)
+                if (x[u] != 0xFFFFFFFFu) atomicAdd(&h[x[u] >> shift], 1u);
+        }
+        __syncthreads();
+        if (gb >= g1 || qb >= nseg) break;
+        qa = qb;
+    }
     __syncthreads();
     for (uint32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) S.hist[(uint64_t)p * S.nwg + blockIdx.x] = h[p];
 }
 
-// The block's events in rounds of blockDim.x consecutive positions: every thread runs the same
-// number of rounds (f gets ok = false past the end), so f may contain block barriers.
-template <bool WITH_MULT, typename F>
+// The block's events in rounds of EPT * blockDim.x consecutive positions, EPT per thread: every
+// thread runs the same number of rounds (f gets ok = false past the end), so f may contain block
+// barriers.
+template <bool WITH_MULT, uint32_t EPT, typename F>
 __device__ void for_block_rounds(const Step& S, SegCache& c, F&& f) {
     const uint32_t nseg = S.seg_count[0];
     const uint64_t D = S.seg_count[1];
@@ -331,11 +389,21 @@ __device__ void for_block_rounds(const Step& S, SegCache& c, F&& f) {
         __syncthreads();
         const uint32_t ga = max(g0, c.off[0]);
         const uint32_t gb = min(g1, c.off[nq]);
-        for (uint32_t gr = ga; gr < gb; gr += blockDim.x) {
-            const uint32_t g = gr + threadIdx.x;
-            const bool ok = g < gb;
-            const uint32_t k = ok ? seg_find(c.off, nq, g) : 0u;
-            f(ok, ok ? c.v0[k] : 0ull, ok ? c.off[k] : 0u, (ok && WITH_MULT) ? c.mult[k] : 0u, ok ? g - c.off[k] : 0u);
+        for (uint32_t gr = ga; gr < gb; gr += EPT * blockDim.x) {
+            bool ok[EPT];
+            uint64_t v0[EPT];
+            uint32_t o[EPT], m[EPT], ii[EPT];
+#pragma unroll
+            for (uint32_t u = 0; u < EPT; ++u) {
+                const uint32_t g = gr + u * blockDim.x + threadIdx.x;
+                ok[u] = g < gb;
+                const uint32_t k = ok[u] ? seg_find(c.off, nq, g) : 0u;
+                v0[u] = ok[u] ? c.v0[k] : 0ull;
+                o[u] = ok[u] ? c.off[k] : 0u;
+                m[u] = (ok[u] && WITH_MULT) ? c.mult[k] : 0u;
+                ii[u] = ok[u] ? g - c.off[k] : 0u;
+            }
+            f(ok, v0, o, m, ii);
         }
         __syncthreads();
         if (gb >= g1 || qb >= nseg) break;
@@ -349,8 +417,9 @@ __device__ void for_block_rounds(const Step& S, SegCache& c, F&& f) {
 // as one coalesced chunk (a record that finds its partition's buffer full is written straight to
 // its place), so the bucket writes leave the workgroup as whole segments instead of one scattered
 // store per event. Positions within a run are unordered; the fold orders by g.
-template <bool WIDE, bool CODE_HERE, bool CODE2>
+constexpr uint32_t kScatterEpt = 2;  // events per thread per round
 // at most 64 VGPRs: two 1024-thread workgroups per CU (8 waves per SIMD)
+template <bool WIDE, bool CODE_HERE, bool CODE2>
 __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8))) k_pscatter(Step S) {
     typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
     extern __shared__ uint32_t dyn[];
@@ -373,19 +442,24 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
     uint32_t err = 0;
-    for_block_rounds<CODE_HERE>(S, cache, [&](bool ok, uint64_t v0, uint32_t g0, uint32_t mult, uint32_t i) {
-        if (ok) {
-            uint32_t x;
-            E e;
-            err |= make_event<WIDE, CODE_HERE, CODE2>(S, v0, g0, mult, i, T, x, e);
-            const uint16_t tl = (uint16_t)(x & mask);
-            const uint32_t p = x >> shift;
+    for_block_rounds<CODE_HERE, kScatterEpt>(S, cache, [&](const bool* ok, const uint64_t* v0, const uint32_t* g0,
+                                                          const uint32_t* mult, const uint32_t* i) {
+        uint32_t x[kScatterEpt];
+        E e[kScatterEpt];
+#pragma unroll
+        for (uint32_t u = 0; u < kScatterEpt; ++u)
+            if (ok[u]) err |= make_event<WIDE, CODE_HERE, CODE2>(S, v0[u], g0[u], mult[u], i[u], T, x[u], e[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < kScatterEpt; ++u) {
+            if (!ok[u]) continue;
+            const uint16_t tl = (uint16_t)(x[u] & mask);
+            const uint32_t p = x[u] >> shift;
             const uint32_t slot = atomicAdd(&cnt[p], 1u);
             if (slot < ch) {
-                buf[p * ch + slot] = e;
+                buf[p * ch + slot] = e[u];
                 buft[p * ch + slot] = tl;
             } else {
-                out[base[p] + slot] = e;
+                out[base[p] + slot] = e[u];
                 outt[base[p] + slot] = tl;
             }
         }
@@ -1057,9 +1131,8 @@ template <class Src>
 void launch_scan(snn_net* net, Src src, uint32_t* out, uint32_t* total, uint64_t cap, uint32_t err_flag,
                  uint32_t* zero_on_overflow) {
     hipLaunchKernelGGL(k_scan_reduce<Src>, dim3(kScanBlocks), dim3(kBlock), 0, net->stream, src, net->partials);
-    hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(kBlock), 0, net->stream, net->partials, (uint32_t)kScanBlocks, total, cap,
-                       err_flag, &net->S.ctl[1], zero_on_overflow);
-    hipLaunchKernelGGL(k_scan_down<Src>, dim3(kScanBlocks), dim3(kBlock), 0, net->stream, src, net->partials, out);
+    hipLaunchKernelGGL(k_scan_down<Src>, dim3(kScanBlocks), dim3(kBlock), 0, net->stream, src, (const uint32_t*)net->partials,
+                       out, total, cap, err_flag, &net->S.ctl[1], zero_on_overflow);
 }
 
 // LDS of the scatter besides its segment cache: base and count per partition plus the
