@@ -276,41 +276,6 @@ __device__ __forceinline__ uint32_t seg_find(const uint32_t* off, uint32_t n, ui
     return lo - 1;
 }
 
-// Calls f(v0, g0, mult, i) for every event of this workgroup's range. Every thread of the block
-// must call it (it synchronises).
-template <bool WITH_MULT, typename F>
-__device__ void for_block_events(const Step& S, SegCache& c, F&& f) {
-    const uint32_t nseg = S.seg_count[0];
-    const uint64_t D = S.seg_count[1];
-    const uint32_t g0 = (uint32_t)(D * blockIdx.x / gridDim.x);
-    const uint32_t g1 = (uint32_t)(D * (blockIdx.x + 1) / gridDim.x);
-    __shared__ uint32_t qstart;
-    if (threadIdx.x == 0 && g0 < g1) qstart = seg_find(S.seg_off, nseg, g0);
-    __syncthreads();
-    if (g0 >= g1) return;
-    uint32_t qa = qstart;
-    while (true) {
-        const uint32_t qb = min(nseg, qa + kSegChunk);
-        const uint32_t nq = qb - qa;
-        for (uint32_t k = threadIdx.x; k < nq; k += blockDim.x) {
-            c.off[k] = S.seg_off[qa + k];
-            c.v0[k] = S.seg_v0[qa + k];
-            if (WITH_MULT) c.mult[k] = S.seg_mult[qa + k];
-        }
-        if (threadIdx.x == 0) c.off[nq] = qb < nseg ? S.seg_off[qb] : (uint32_t)D;
-        __syncthreads();
-        const uint32_t ga = max(g0, c.off[0]);
-        const uint32_t gb = min(g1, c.off[nq]);
-        for (uint32_t g = ga + threadIdx.x; g < gb; g += blockDim.x) {
-            const uint32_t k = seg_find(c.off, nq, g);
-            f(c.v0[k], c.off[k], WITH_MULT ? c.mult[k] : 0u, g - c.off[k]);
-        }
-        __syncthreads();
-        if (gb >= g1 || qb >= nseg) break;
-        qa = qb;
-    }
-}
-
 // Per-workgroup histogram of the tick's events over target partitions (LDS atomics only).
 __global__ void __launch_bounds__(kPBlock) k_pcount(Step S) {
     extern __shared__ uint32_t h[];
