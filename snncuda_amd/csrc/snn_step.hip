@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(kBlock) k_segments(Step S) {
 // Event ranges: workgroup w owns events [D*w/nwg, D*(w+1)/nwg) (a contiguous range of g), caches the
 // descriptors of the segments covering it in LDS and lets its lanes walk consecutive events, so the
 // CSR reads are coalesced and the load is balanced whatever the fan-out distribution.
-constexpr uint32_t kSegChunk = 2048;
+constexpr uint32_t kSegChunk = 1024;
 
 struct SegCache {
     uint32_t off[kSegChunk + 1];
@@ -1100,9 +1100,9 @@ void launch_scan(snn_net* net, Src src, uint32_t* out, uint32_t* total, uint64_t
                        out, total, cap, err_flag, &net->S.ctl[1], zero_on_overflow);
 }
 
-// LDS of the scatter besides its segment cache: base and count per partition plus the
-// write-combining buffers
-constexpr uint32_t kScatterDynLds = 120 * 1024;
+// LDS of the scatter besides its 16 KB segment cache: base and count per partition plus the
+// write-combining buffers; two scatter workgroups fit in a CU's 160 KB
+constexpr uint32_t kScatterDynLds = 64 * 1024;
 
 template <bool WIDE, bool CODE_HERE, bool CODE2>
 void launch_scatter(snn_net* net) {
