@@ -4,7 +4,8 @@
 Workload (default): the 1,000,000-neuron network with lognormal fan-out (median 1,500, sigma 0.8,
 cap 20,000; ~2.07e9 synapses), 80/20 E/I, delays U{1..32}, Basal/Apical compartments, one code
 offset per synapse, 10,000 Poisson input neurons at p=0.005 with fan-out 1,000, STDP off
-(``--stdp`` for on). Pinned free parameters: threshold 2.5, excitatory weights U[0,0.5),
+(``--stdp`` for on, on every synapse but those of inhibitory neurons). Pinned free parameters:
+threshold 2.5 (2.75 with STDP), excitatory weights U[0,0.5),
 inhibitory GABA_A weights U[0,5), and a 2% ignition inject at tick 0; this keeps ~0.5% of the
 neurons firing per tick (~1e7 delivered events per tick) for the whole run.
 
@@ -76,6 +77,12 @@ def parse():
     return ap.parse_args()
 
 
+def threshold_of(args) -> float:
+    # STDP on potentiates the recurrent excitation: at 2.5 the 1M-neuron network runs away within
+    # 500 ticks, at 2.75 it holds ~0.37% of the neurons firing per tick for 2,200 ticks
+    return 2.75 if args.stdp else 2.5
+
+
 def cpu_baseline(args):
     """Oracle, one thread, on the same law at --cpu-neurons neurons."""
     import numpy as np
@@ -83,10 +90,10 @@ def cpu_baseline(args):
     from snncuda_amd import workloads as W
     n = args.cpu_neurons
     net = W.ei_network(n, W.lognormal_fanout(0, n), 0, n_sources=max(1, n // 100), source_fanout=1000,
-                       p_source=0.005, ticks=args.cpu_ticks, threshold=2.5, delay_max=32,
-                       compartments=(W.BASAL, W.APICAL), plastic=args.stdp, w_inh=5.0)
+                       p_source=0.005, ticks=args.cpu_ticks, threshold=threshold_of(args), delay_max=32,
+                       compartments=(W.BASAL, W.APICAL), plastic="exc" if args.stdp else False, w_inh=5.0)
     hit = np.nonzero(W.unif24(W.hash3(0, 40, np.arange(n, dtype=np.uint64))) < np.float32(0.02))[0]
-    cmds = [(0, int(i), 0, 2.5) for i in hit] + net.commands
+    cmds = [(0, int(i), 0, threshold_of(args)) for i in hit] + net.commands
     o = oracle.Oracle(net)
     n_inj = 0
     for k, a, t, w in cmds:
@@ -294,8 +301,10 @@ def main():
     from snncuda_amd import workloads as W
 
     total = args.warmup + 2 * args.steps
-    spec, cmds = W.config2_spec(0, n=args.neurons, ticks=total, threshold=2.5, w_inh=5.0, kick=0.02,
-                                plastic=args.stdp)
+    # STDP on: every synapse but those of inhibitory neurons is plastic (the rule caps weights at
+    # 2.0, below the pinned inhibitory magnitudes U[0,5), so plastic inhibition would collapse)
+    spec, cmds = W.config2_spec(0, n=args.neurons, ticks=total, threshold=threshold_of(args), w_inh=5.0, kick=0.02,
+                                plastic="exc" if args.stdp else False)
     injects = [c for c in cmds if c[0] == 0]
     inj_ticks = np.array([t for _, _, t, _ in injects], dtype=np.int64)
     with sa.Network(spec=spec, max_events_per_tick=1 << 27, fire_log_capacity=1 << 26,
@@ -334,8 +343,8 @@ def main():
 
     def traffic_of(name):
         # counters scaled by the copy-kernel calibration (profiles/copy_calibration.txt), per
-        # delivered event, times this run's events per launch
-        f = ROOT / "profiles" / name
+        # delivered event, times this run's events per launch (files *_stdp.json for STDP on)
+        f = ROOT / "profiles" / (name.replace(".json", "_stdp.json") if args.stdp else name)
         if not f.exists():
             return None
         tr = json.loads(f.read_text())
@@ -374,13 +383,14 @@ def main():
         "data": "synthetic (seeded generator, built on the device)",
         "config": {
             "workload": f"{args.neurons}-neuron lognormal fan-out net (median 1500, sigma 0.8, cap 20000), "
-                        f"delays U{{1..32}}, 2 compartments, codes {{0}}, STDP {'on' if args.stdp else 'off'}",
+                        f"delays U{{1..32}}, 2 compartments, codes {{0}}, "
+                        f"STDP {'on (excitatory synapses)' if args.stdp else 'off'}",
             "neurons": args.neurons + spec["n_sources"],
             "synapses": int(E),
             "events_per_tick": float(dpt.mean()),
             "fires_per_tick": float(fpt.mean()),
             "fire_fraction_per_tick": float(fpt.mean()) / args.neurons,
-            "threshold": 2.5,
+            "threshold": threshold_of(args),
             "parallelism": "1 GPU",
         },
         "roofline": fold_rl,

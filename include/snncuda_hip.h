@@ -93,7 +93,8 @@ typedef struct snn_ei_desc {
     uint32_t delay_max;         /* <= 63 */
     uint32_t n_comps;
     uint8_t comps[4];
-    int32_t plastic;
+    int32_t plastic;            /* 0: no plastic synapses, 1: every synapse, 2: every synapse except
+                                   those of inhibitory neurons [n_exc, n) */
     float threshold;
     float source_threshold;
     uint32_t window;

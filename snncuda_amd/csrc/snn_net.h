@@ -249,7 +249,7 @@ inline HostNet build_host_net(const NetDesc& d) {
                 if (idx == 256) throw std::invalid_argument("more than 256 distinct max_weight values on plastic synapses");
                 h.wmax.push_back(mx);
             }
-            StdpRec r;
+            StdpRec r{};
             r.weight = d.weight[j];
             r.last_pre = kNone;
             r.last_post = kNone;

@@ -531,11 +531,14 @@ SNN_HD uint32_t code_observeN(CodeRecN& r, uint32_t tick, uint32_t times, uint32
 //   aux bits 0-7: index of min(2.0f, max_weight) in the network's wmax table (StdpRule::apply's
 //   `maximum`); bits 8-31: post-synaptic updates already applied at tick last_post (only meaningful
 //   while last_post is the current tick).
-struct StdpRec {           // 16 B
+struct alignas(32) StdpRec {  // 32 B: a synapse's STDP state and counters in one cache-line access
     float weight;
     uint32_t last_pre;     // kNone when unset
     uint32_t last_post;    // kNone when unset
     uint32_t aux;
+    uint32_t upd;          // changed updates (plasticity_update_count)
+    uint32_t ltp;          // of which with delta > 0
+    uint32_t pad[2];
 };
 
 // StdpRule::apply: clamp(w + delta, 0, min(2, max_w)), std::clamp written out as libstdc++ does.
@@ -552,6 +555,10 @@ struct StdpTally {
     SNN_HD void add(uint32_t c) {
         n += c & 1u;
         ltp += c >> 16;
+    }
+    SNN_HD void store(StdpRec& r) const {
+        r.upd += n;
+        r.ltp += ltp;
     }
 };
 SNN_HD uint32_t stdp_apply(StdpRec& r, uint32_t pre, uint32_t post, const float* ltp, const float* ltd,

@@ -47,7 +47,8 @@ struct Entry16 {
     uint32_t gk;
     float w;               // weight; reused as a scratch word once the event is applied
     uint32_t tpos;         // STDP record (kNone: not plastic or external)
-    uint32_t slot;         // synapse slot (kNone: external)
+    uint32_t slot;         // synapse slot (kNone: external); with the code patterns advanced in the
+                           // scatter (CODE_HERE) the synapse's delay instead
 };
 
 struct Group {             // a run of the tick's due list
@@ -77,8 +78,6 @@ struct Step {
     const uint32_t* tpos;
     const uint64_t* in_ptr;
     StdpRec* stdp;
-    uint32_t* upd_count;   // changed STDP updates per plastic synapse (plasticity_update_count)
-    uint32_t* ltp_count;   // of which with delta > 0
     const float* wmax;           // wmax table indexed by StdpRec::aux bits 0-7
     const NeuronParams* np;
     uint32_t tp_uniform;         // every neuron has the same window / similarity / max_refs
@@ -251,7 +250,7 @@ SNN_HD uint32_t make_event(const Step& S, uint64_t v0, uint32_t g0, uint32_t mul
     e.gk = (g << 4) | (m & 15u);
     if constexpr (WIDE) {
         e.tpos = (m & 16u) ? S.tpos[slot] : kNone;
-        e.slot = slot;
+        e.slot = CODE_HERE ? (uint32_t)S.delay[slot] : slot;
     }
     if (CODE_HERE) {
         if (mult && !S.code_lazy) {
@@ -317,11 +316,6 @@ SNN_HD StdpTally post_catch_up(const Step& S, StdpRec& r, uint32_t T, uint32_t f
     return changed;
 }
 
-SNN_HD void tally_store(const Step& S, uint64_t t, StdpTally c) {
-    if (c.n) S.upd_count[t] += c.n;
-    if (c.ltp) S.ltp_count[t] += c.ltp;
-}
-
 // The sequential fold of target x's k events at tick T (ev is the target's own array, in any
 // order; it is sorted by g here). Returns error flags.
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2, typename E>
@@ -348,12 +342,12 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
                 // first the post-synaptic updates of this tick's earlier fires of x
                 StdpTally changed = post_catch_up(S, r, T, fires);
                 // pre-synaptic side: process_synaptic_input (:95-126); the code offset is ignored
-                const uint32_t dl = S.delay[e.slot];
+                const uint32_t dl = CODE_HERE ? e.slot : S.delay[e.slot];
                 const uint32_t pre = T >= dl ? T - dl : T;
                 if (r.last_post != kNone) changed.add(stdp_apply(r, pre, r.last_post, S.ltp, S.ltd, S.wmax));
                 r.last_pre = pre;
+                changed.store(r);
                 S.stdp[e.tpos] = r;
-                tally_store(S, e.tpos, changed);
                 w = r.weight;
             }
             if (!CODE_HERE && e.slot != kNone) {
@@ -401,9 +395,8 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
                 const uint32_t t = ev[i].tpos;
                 if (t == kNone) continue;
                 StdpRec r = S.stdp[t];
-                const StdpTally changed = post_catch_up(S, r, T, fires);
+                post_catch_up(S, r, T, fires).store(r);
                 S.stdp[t] = r;
-                tally_store(S, t, changed);
             }
         }
     }
@@ -416,9 +409,8 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
 SNN_HD void post_scan_record(const Step& S, uint64_t t, uint32_t T, uint32_t m) {
     StdpRec r = S.stdp[t];
     if (r.last_post == T && (r.aux >> 8) == m) return;  // done inside the fold
-    const StdpTally changed = post_catch_up(S, r, T, m);
+    post_catch_up(S, r, T, m).store(r);
     S.stdp[t] = r;
-    tally_store(S, t, changed);
 }
 
 }  // namespace snn

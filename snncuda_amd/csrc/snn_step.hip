@@ -626,6 +626,14 @@ __global__ void k_advance(Step S, uint32_t t0) {
     S.ctl[0] = T + 1;
 }
 
+// the STDP records' update counters, record order
+__global__ void k_stdp_counts(const StdpRec* stdp, uint64_t nt, uint32_t* upd, uint32_t* ltp) {
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < nt; t += (uint64_t)gridDim.x * blockDim.x) {
+        upd[t] = stdp[t].upd;
+        ltp[t] = stdp[t].ltp;
+    }
+}
+
 // readback gathers (connectome order)
 __global__ void k_gather_weights(const uint32_t* slot_of, const float* w, const uint32_t* tpos, const StdpRec* stdp,
                                  uint64_t e, float* out) {
@@ -819,6 +827,12 @@ __device__ __forceinline__ uint32_t gen_target(const GenNet& N, uint64_t s, uint
     return gbelow(ghash(N.key[2], ctr), N.n);
 }
 
+// plastic synapses of a generated network (snn_ei_desc::plastic): none, all, or all but those of
+// inhibitory neurons; s is the network-local source
+__device__ __forceinline__ bool gen_plastic(const GenNet& N, uint64_t s) {
+    return N.plastic == 1 || (N.plastic == 2 && !(s < N.n && s >= N.n_exc));
+}
+
 // edge j of network-local source s
 __device__ GenEdge gen_edge(const GenNet& N, uint64_t s, uint64_t j) {
     const uint64_t ctr = (s << 24) + j;
@@ -837,7 +851,7 @@ __device__ GenEdge gen_edge(const GenNet& N, uint64_t s, uint64_t j) {
         rec = kAmpa;
         comp = kBasal;
     }
-    g.meta = (uint8_t)(rec | (comp << 2) | ((N.plastic ? 1u : 0u) << 4));
+    g.meta = (uint8_t)(rec | (comp << 2) | ((gen_plastic(N, s) ? 1u : 0u) << 4));
     return g;
 }
 
@@ -900,7 +914,7 @@ __global__ void __launch_bounds__(kBlock) k_gen_count(GenParams P) {
         const uint64_t ctr = (sl << 24) + j;
         const uint32_t d = gen_delay(N, sl, ctr);
         for (uint32_t c = 0; c < N.ncode; ++c) atomicAdd(&h[gen_lag(d, gen_code(N, ctr, c))], 1u);
-        if (N.plastic) atomicAdd(&P.incnt[N.noff + gen_target(N, sl, ctr)], 1u);
+        if (gen_plastic(N, sl)) atomicAdd(&P.incnt[N.noff + gen_target(N, sl, ctr)], 1u);
     }
     wave_lds_sync();
     wave_bins_excl_scan(h, lane);
@@ -949,10 +963,10 @@ __global__ void __launch_bounds__(kBlock) k_gen_place(GenParams P) {
             P.delay[slot] = (uint16_t)g.delay;
             if (P.expected) P.expected[slot] = (uint8_t)(N.ncode > 2 ? N.ncode : 2);
             P.slot_of[r0 + j] = (uint32_t)slot;
-            if (N.plastic) {
+            if (g.meta & 16u) {
                 const uint64_t tp = P.in_ptr[g.tgt] + atomicAdd(&P.incnt[g.tgt], 1u);
                 P.tpos[slot] = (uint32_t)tp;
-                StdpRec r;
+                StdpRec r{};
                 r.weight = g.w;
                 r.last_pre = kNone;
                 r.last_post = kNone;
@@ -1624,8 +1638,6 @@ std::unique_ptr<snn_net> create_net(const NetDesc& d, const snn_options& o) {
         S.tpos = dev_upload(own, h.tpos);
         S.in_ptr = dev_upload(own, h.in_ptr);
         S.stdp = dev_upload(own, h.stdp);
-        S.upd_count = dev_alloc<uint32_t>(own, h.stdp.size());
-        S.ltp_count = dev_alloc<uint32_t>(own, h.stdp.size());
         S.wmax = dev_upload(own, h.wmax);
         net->tpos_conn_dev = dev_upload(own, h.tpos_conn);
         net->n_tpos = h.stdp.size();
@@ -1672,7 +1684,8 @@ std::unique_ptr<snn_net> generate_nets(const snn_ei_desc* g, uint64_t m, const s
         for (int c = 0; c < 4; ++c) N.comps[c] = d.comps[c];
         N.ncode = d.n_codes ? d.n_codes : 1;
         N.code_max = d.code_max;
-        N.plastic = d.plastic ? 1u : 0u;
+        if (d.plastic < 0 || d.plastic > 2) throw std::invalid_argument("plastic must be 0, 1 or 2");
+        N.plastic = (uint32_t)d.plastic;
         noff[k + 1] = noff[k] + N.nall;
         if (noff[k + 1] >= 0xFFFFFFFFull) throw std::invalid_argument("too many neurons");
         lmax = std::max<uint32_t>(lmax, d.delay_max + d.code_max);
@@ -1755,8 +1768,6 @@ std::unique_ptr<snn_net> generate_nets(const snn_ei_desc* g, uint64_t m, const s
         gp.tpos_conn = dev_alloc<uint32_t>(own, n_tpos, false);
         S.tpos = gp.tpos;
         S.stdp = gp.stdp;
-        S.upd_count = dev_alloc<uint32_t>(own, n_tpos);
-        S.ltp_count = dev_alloc<uint32_t>(own, n_tpos);
         std::vector<float> wm(1, 2.0f);  // max_weight 2.0 on every synapse: min(2, 2)
         S.wmax = dev_upload(own, wm);
         net->tpos_conn_dev = gp.tpos_conn;
@@ -1786,6 +1797,24 @@ std::unique_ptr<snn_net> generate_nets(const snn_ei_desc* g, uint64_t m, const s
 void snn_internal_set_error(const std::string& msg) { g_last_error = msg; }
 
 // ------------------------------------------------------------------------------------ C ABI
+
+namespace {
+
+// per STDP record (record order): changed updates and those with delta > 0
+void read_stdp_counts(snn_net* net, uint32_t* upd, uint32_t* ltp) {
+    const uint64_t nt = net->n_tpos;
+    std::vector<DevBuf> tmp;
+    uint32_t* du = dev_alloc<uint32_t>(tmp, nt, false);
+    uint32_t* dl = dev_alloc<uint32_t>(tmp, nt, false);
+    const uint64_t nb = std::min<uint64_t>((nt + kBlock - 1) / kBlock, 65536);
+    hipLaunchKernelGGL(k_stdp_counts, dim3((uint32_t)nb), dim3(kBlock), 0, net->stream, (const StdpRec*)net->S.stdp, nt, du, dl);
+    check(hipGetLastError(), "k_stdp_counts");
+    check(hipMemcpyAsync(upd, du, nt * 4, hipMemcpyDeviceToHost, net->stream), "read stdp");
+    check(hipMemcpyAsync(ltp, dl, nt * 4, hipMemcpyDeviceToHost, net->stream), "read stdp");
+    check(hipStreamSynchronize(net->stream), "read stdp");
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -2323,9 +2352,9 @@ int snn_net_read_synapses(snn_net* net, float* weight, uint64_t* pre_count, uint
             std::fill(update_count, update_count + e, 0);
             if (net->n_tpos) {
                 const uint64_t nt = net->n_tpos;
-                std::vector<uint32_t> upd(nt);
+                std::vector<uint32_t> upd(nt), ltp(nt);
                 std::vector<uint32_t> conn(nt);
-                check(hipMemcpy(upd.data(), net->S.upd_count, nt * 4, hipMemcpyDeviceToHost), "read stdp");
+                read_stdp_counts(net, upd.data(), ltp.data());
                 check(hipMemcpy(conn.data(), net->tpos_conn_dev, nt * 4, hipMemcpyDeviceToHost), "read stdp");
                 for (uint64_t t = 0; t < nt; ++t) update_count[conn[t]] = upd[t];
             }
@@ -2346,8 +2375,7 @@ int snn_net_read_plasticity(snn_net* net, uint64_t* ltp_updates, uint64_t* ltd_u
         const uint64_t nt = net->n_tpos;
         std::vector<uint32_t> upd(nt), ltp(nt), conn(nt);
         check(hipStreamSynchronize(net->stream), "sync");
-        check(hipMemcpy(upd.data(), net->S.upd_count, nt * 4, hipMemcpyDeviceToHost), "read stdp");
-        check(hipMemcpy(ltp.data(), net->S.ltp_count, nt * 4, hipMemcpyDeviceToHost), "read stdp");
+        read_stdp_counts(net, upd.data(), ltp.data());
         check(hipMemcpy(conn.data(), net->tpos_conn_dev, nt * 4, hipMemcpyDeviceToHost), "read stdp");
         for (uint64_t t = 0; t < nt; ++t) {
             if (ltp_updates) ltp_updates[conn[t]] = ltp[t];

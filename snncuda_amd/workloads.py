@@ -157,9 +157,20 @@ def _fanout_edges(seed: int, stream: int, sources: np.ndarray, fanout: np.ndarra
     return src, t + target_base, ctr
 
 
+PLASTIC_MODES = {False: 0, True: 1, "exc": 2}  # snn_ei_desc::plastic
+
+
+def plastic_flags(mode: int, src: np.ndarray, n: int, n_exc: int) -> np.ndarray:
+    """Per-synapse plastic flags of ei_network's plasticity mode (0 none, 1 all, 2 all but the
+    synapses of inhibitory neurons [n_exc, n))."""
+    if mode == 2:
+        return (~((src >= n_exc) & (src < n))).astype(np.uint8)
+    return np.full(src.shape[0], 1 if mode == 1 else 0, np.uint8)
+
+
 def ei_network(n: int, fanout, seed: int, *, n_sources: int, source_fanout: int, p_source: float,
                ticks: int, threshold: float, delay_max: int, compartments: Tuple[int, ...],
-               n_codes: int = 1, code_max: int = 0, plastic: bool = True, exc_frac: float = 0.8,
+               n_codes: int = 1, code_max: int = 0, plastic=True, exc_frac: float = 0.8,
                w_exc: float = 0.5, w_inh: float = 1.0, window: int = 32, sim_thr: float = 0.9,
                max_refs: int = 1, name: str = "") -> Net:
     """E/I network of ``n`` neurons plus ``n_sources`` Poisson input neurons.
@@ -167,13 +178,15 @@ def ei_network(n: int, fanout, seed: int, *, n_sources: int, source_fanout: int,
     Neurons [0, n*exc_frac) are excitatory (AMPA, weight U[0, w_exc)), the rest inhibitory
     (GABA_A, weight magnitude U[0, w_inh)). Compartments are drawn uniformly from
     ``compartments``; delays from U{1..delay_max}; code offsets (``n_codes`` per synapse) from
-    U{0..code_max} in draw order. Source s fires at tick t when hash < p_source.
+    U{0..code_max} in draw order. Source s fires at tick t when hash < p_source. ``plastic``:
+    False (no STDP), True (every synapse) or "exc" (every synapse but those of inhibitory neurons).
     """
+    plastic_mode = PLASTIC_MODES[plastic]
     n_exc = int(round(n * exc_frac))
     fan = np.broadcast_to(np.asarray(fanout, dtype=np.int64), (n,)).copy()
     spec = dict(n=n, n_sources=n_sources, fanout=fan, source_fanout=source_fanout, seed=seed, n_exc=n_exc,
-                w_exc=w_exc, w_inh=w_inh, delay_max=delay_max, compartments=tuple(compartments), plastic=plastic,
-                threshold=threshold, window=window, sim_thr=sim_thr, max_refs=max_refs,
+                w_exc=w_exc, w_inh=w_inh, delay_max=delay_max, compartments=tuple(compartments),
+                plastic=plastic_mode, threshold=threshold, window=window, sim_thr=sim_thr, max_refs=max_refs,
                 n_codes=max(1, n_codes), code_max=code_max)
     src, tgt, ctr = _fanout_edges(seed, 1, np.arange(n), fan, n, 0, True)
     ssrc, stgt, sctr = _fanout_edges(seed, 2, np.arange(n, n + n_sources), np.full(n_sources, source_fanout), n, 0, False)
@@ -209,7 +222,7 @@ def ei_network(n: int, fanout, seed: int, *, n_sources: int, source_fanout: int,
         np.concatenate([delay, sdelay]),
         np.concatenate([comp, np.full(ssrc.shape[0], BASAL, np.uint8)]),
         np.concatenate([receptor, np.full(ssrc.shape[0], AMPA, np.uint8)]),
-        np.full(e, 1 if plastic else 0, np.uint8), code_ptr=code_ptr, codes=codes, name=name,
+        plastic_flags(plastic_mode, all_src, n, n_exc), code_ptr=code_ptr, codes=codes, name=name,
         info=dict(n=n, n_sources=n_sources, e_net=int(e_net), seed=seed, ticks=ticks,
                   threshold=threshold, p_source=p_source, spec=spec if n_codes >= 1 or code_max == 0 else None),
     )
@@ -347,16 +360,16 @@ def config2(seed: int = 0, n: int = 1_000_000, n_sources: int = 10_000, source_f
 
 
 def config2_spec(seed: int = 0, n: int = 1_000_000, n_sources: int = 10_000, source_fanout: int = 1000,
-                 ticks: int = 2000, threshold: float = 8.0, plastic: bool = False, w_exc: float = 0.5,
+                 ticks: int = 2000, threshold: float = 8.0, plastic=False, w_exc: float = 0.5,
                  w_inh: float = 1.0, kick: float = 0.0):
     """The config-2 network as a device-generator spec plus its input commands, without building the
     synapse arrays on the host (equal to config2(...) built there). ``kick`` > 0 adds, before the
     Poisson input, an inject of weight ``threshold`` at tick 0 into each network neuron with
-    probability ``kick``."""
+    probability ``kick``. ``plastic``: False, True or "exc" (as ei_network)."""
     fan = lognormal_fanout(seed, n)
     spec = dict(n=n, n_sources=n_sources, fanout=fan, source_fanout=source_fanout, seed=seed, n_exc=int(round(n * 0.8)),
-                w_exc=w_exc, w_inh=w_inh, delay_max=32, compartments=(BASAL, APICAL), plastic=plastic,
-                threshold=threshold, window=32, sim_thr=0.9, max_refs=1)
+                w_exc=w_exc, w_inh=w_inh, delay_max=32, compartments=(BASAL, APICAL),
+                plastic=PLASTIC_MODES[plastic], threshold=threshold, window=32, sim_thr=0.9, max_refs=1)
     cmds = poisson_commands(seed, n, n_sources, 0.005, ticks)
     if kick > 0:
         hit = np.nonzero(unif24(hash3(seed, 40, np.arange(n, dtype=np.uint64))) < np.float32(kick))[0]
@@ -393,7 +406,8 @@ def config3_spec(seed: int, ticks: int = 2000, threshold: float = 3.0, n_range=(
     n, f = config3_shape(seed, n_range, fan_range)
     spec = dict(n=n, n_sources=100, fanout=np.full(n, f, np.int64), source_fanout=50, seed=seed,
                 n_exc=int(round(n * 0.8)), w_exc=0.5, w_inh=w_inh, delay_max=20, compartments=(BASAL, APICAL),
-                plastic=plastic, threshold=threshold, window=32, sim_thr=0.9, max_refs=1, n_codes=2, code_max=7)
+                plastic=PLASTIC_MODES[plastic], threshold=threshold, window=32, sim_thr=0.9, max_refs=1, n_codes=2,
+                code_max=7)
     return spec, poisson_commands(seed, n, 100, 0.01, ticks)
 
 

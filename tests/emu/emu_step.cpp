@@ -30,7 +30,7 @@ struct Emu {
     // owned storage
     std::vector<CodeRec2> code2v;
     std::vector<CodeRecN> codeNv;
-    std::vector<uint32_t> pre_count, upd_count, ltp_count, last_fired;
+    std::vector<uint32_t> pre_count, last_fired;
     std::vector<NeuronDyn> dyn;
     std::vector<TemporalHdr> th;
     std::vector<Runs> cand;
@@ -250,10 +250,6 @@ void* emu_create(uint64_t n, const float* thr, const uint64_t* win, const float*
             S.tpos = h.tpos.data();
             S.in_ptr = h.in_ptr.data();
             S.stdp = h.stdp.data();
-            m->upd_count.assign(h.stdp.size() + 1, 0);
-            S.upd_count = m->upd_count.data();
-            m->ltp_count.assign(h.stdp.size() + 1, 0);
-            S.ltp_count = m->ltp_count.data();
             S.wmax = h.wmax.data();
         }
         S.np = h.params.data();

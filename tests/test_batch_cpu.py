@@ -45,3 +45,20 @@ def test_union_model_equals_each_network_alone():
         es = slice(int(eoff[k]), int(eoff[k + 1]))
         for key in ("weight", "pre_count", "c_match", "c_learned"):
             assert np.array_equal(us[key][es], rs[key].astype(us[key].dtype)), (net.name, key)
+
+
+def test_plasticity_modes_of_the_ei_family():
+    # plastic="exc": every synapse is plastic except those of inhibitory network neurons
+    n = 300
+    net = W.ei_network(n, 20, 1, n_sources=10, source_fanout=5, p_source=0.01, ticks=10, threshold=2.0,
+                       delay_max=5, compartments=(W.BASAL,), plastic="exc", w_inh=5.0)
+    n_exc = net.info["spec"]["n_exc"]
+    inh = (net.src >= n_exc) & (net.src < n)
+    assert inh.any() and (~inh).any()
+    assert np.array_equal(net.plastic, (~inh).astype(np.uint8))
+    assert np.array_equal(net.receptor[net.src < n] == W.GABA_A, inh[net.src < n])
+    assert net.info["spec"]["plastic"] == 2
+    for mode, flag in ((False, 0), (True, 1)):
+        m = W.ei_network(n, 20, 1, n_sources=10, source_fanout=5, p_source=0.01, ticks=10, threshold=2.0,
+                         delay_max=5, compartments=(W.BASAL,), plastic=mode)
+        assert (m.plastic == flag).all() and m.info["spec"]["plastic"] == flag

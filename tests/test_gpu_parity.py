@@ -77,7 +77,7 @@ def test_split_runs_equal_one_run():
     b.close()
 
 
-@pytest.mark.parametrize("plastic", [False, True])
+@pytest.mark.parametrize("plastic", [False, True, "exc"])
 def test_device_generator_equals_host_build(plastic):
     # snn_net_create_ei must build the same network as snn_net_create on workloads.config2's arrays
     net = W.ei_network(3000, W.lognormal_fanout(3, 3000), 3, n_sources=60, source_fanout=200, p_source=0.05,
