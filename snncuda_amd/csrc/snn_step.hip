@@ -456,9 +456,10 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
     if (err) atomicOr(&S.ctl[1], err);
 }
 
-// LDS stage of one fold round; with the 4096-neuron partition counters (32 KB) the fold uses
-// all but 3 KB of the 160 KB of LDS a workgroup may hold
-constexpr uint32_t kStageBytes = 122880;
+// LDS stage of one fold round; with the 4096-neuron partition counters (32 KB) and the
+// touched-target list (8 KB) the fold uses all but 3 KB of the 160 KB of LDS a workgroup may hold
+constexpr uint32_t kStageBytes = 114688;
+constexpr uint32_t kMaxPartition = 4096;  // neurons per partition at most (partition shift <= 12)
 constexpr uint32_t kRound = kPBlock;  // targets folded per round (one per thread)
 
 // One workgroup per target partition. The partition's targets are folded in rounds of 1024 (one
@@ -471,6 +472,7 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
     constexpr uint32_t kCap = kStageBytes / sizeof(E);
     __shared__ E lds_stage[kCap];
     __shared__ uint32_t fill[kRound];
+    __shared__ uint16_t touched[kMaxPartition];  // sparse partitions: the targets with events
     extern __shared__ uint32_t sh[];
     const uint32_t PS = 1u << S.pshift;
     uint32_t* cnt = sh;
@@ -497,6 +499,47 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
             if (tv[u] != kNoTarget) atomicAdd(&cnt[tv[u]], 1u);
     }
     __syncthreads();
+    uint32_t err = 0;
+    if (c <= kCap) {
+        // Sparse partition: every event fits the stage at once, so one round folds the touched
+        // targets only, one per thread. One scan gives both the targets' places (low 16 bits) and
+        // the touched targets' ranks (high 16 bits); c <= kCap < 2^16 keeps the halves apart.
+        uint32_t run = 0;
+        for (uint32_t t0 = 0; t0 < PS; t0 += blockDim.x) {
+            const uint32_t t = t0 + threadIdx.x;
+            const uint32_t v = cnt[t];
+            uint32_t tot;
+            const uint32_t ex = run + block_excl_scan_n<kPBlock>(v | (v ? 1u << 16 : 0u), &tot);
+            off[t] = ex & 0xFFFFu;
+            cnt[t] = ex & 0xFFFFu;  // fill cursor
+            if (v) touched[ex >> 16] = (uint16_t)t;
+            run += tot;
+        }
+        if (threadIdx.x == 0) off[PS] = run & 0xFFFFu;
+        const uint32_t ntouched = run >> 16;
+        __syncthreads();
+        for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * kPBlock) {
+            uint32_t tv[kUnroll];
+            E ev[kUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                const uint32_t i = i0 + u * kPBlock;
+                tv[u] = i < c ? tls[i] : kNoTarget;
+                if (i < c) ev[u] = src[i];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u)
+                if (tv[u] != kNoTarget) lds_stage[atomicAdd(&cnt[tv[u]], 1u)] = ev[u];
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < ntouched; j += blockDim.x) {
+            const uint32_t t = touched[j];
+            const uint32_t x = (p << S.pshift) + t;
+            if (x < S.n) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2>(S, lds_stage + off[t], off[t + 1] - off[t], x, T);
+        }
+        if (err) atomicOr(&S.ctl[1], err);
+        return;
+    }
     uint32_t run = 0;
     for (uint32_t t0 = 0; t0 < PS; t0 += blockDim.x) {
         const uint32_t v = cnt[t0 + threadIdx.x];
@@ -507,7 +550,6 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
     }
     if (threadIdx.x == 0) off[PS] = run;
     __syncthreads();
-    uint32_t err = 0;
     const uint32_t rounds = PS / kRound;
     for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t lo = off[r * kRound], hi = off[(r + 1) * kRound];
