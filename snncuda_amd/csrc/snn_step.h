@@ -38,17 +38,39 @@ constexpr uint64_t kInjFlag = 1ull << 63;
 constexpr uint32_t kGroupInj = 0x80000000u;
 constexpr uint32_t kMaxEventsPerTick = 1u << 28;  // g is packed in 28 bits
 
+SNN_HD uint32_t f2u(float f) {
+    union { float f; uint32_t u; } c;
+    c.f = f;
+    return c.u;
+}
+SNN_HD float u2f(uint32_t u) {
+    union { float f; uint32_t u; } c;
+    c.u = u;
+    return c.f;
+}
+
 // event entries in a target's bin
 struct Entry8 {            // no STDP, code patterns updated edge-parallel
     uint32_t gk;           // g << 4 | compartment << 2 | receptor
     float w;
 };
-struct Entry16 {
+struct Entry12 {           // STDP, code patterns updated edge-parallel
     uint32_t gk;
-    float w;               // weight; reused as a scratch word once the event is applied
+    float w;               // weight; for a plastic event (its weight is in the STDP record) the
+                           // bits of the synapse's delay
     uint32_t tpos;         // STDP record (kNone: not plastic or external)
-    uint32_t slot;         // synapse slot (kNone: external); with the code patterns advanced in the
-                           // scatter (CODE_HERE) the synapse's delay instead
+};
+struct Entry16 {           // code patterns updated per event in the fold
+    uint32_t gk;
+    float w;               // weight
+    uint32_t tpos;         // STDP record (kNone: not plastic or external)
+    uint32_t slot;         // synapse slot (kNone: external)
+};
+// the entry of a step variant: WIDE (STDP or per-event code patterns), CODE_HERE (code patterns
+// advanced in the scatter)
+template <bool WIDE, bool CODE_HERE>
+struct EntryOf {
+    typedef typename std::conditional<!WIDE, Entry8, typename std::conditional<CODE_HERE, Entry12, Entry16>::type>::type type;
 };
 
 struct Group {             // a run of the tick's due list
@@ -236,10 +258,8 @@ SNN_HD uint32_t make_event(const Step& S, uint64_t v0, uint32_t g0, uint32_t mul
         x = S.inj_tgt[k];
         e.w = S.inj_w[k];
         e.gk = (g << 4) | (kSoma << 2) | kAmpa;  // process_external_input: Soma, Ampa
-        if constexpr (WIDE) {
-            e.tpos = kNone;
-            e.slot = kNone;
-        }
+        if constexpr (WIDE) e.tpos = kNone;
+        if constexpr (WIDE && !CODE_HERE) e.slot = kNone;
         return 0;
     }
     const uint64_t v = v0 + i;
@@ -250,7 +270,11 @@ SNN_HD uint32_t make_event(const Step& S, uint64_t v0, uint32_t g0, uint32_t mul
     e.gk = (g << 4) | (m & 15u);
     if constexpr (WIDE) {
         e.tpos = (m & 16u) ? S.tpos[slot] : kNone;
-        e.slot = CODE_HERE ? (uint32_t)S.delay[slot] : slot;
+        if constexpr (CODE_HERE) {
+            if (m & 16u) e.w = u2f(S.delay[slot]);
+        } else {
+            e.slot = slot;
+        }
     }
     if (CODE_HERE) {
         if (mult && !S.code_lazy) {
@@ -292,17 +316,6 @@ SNN_HD void sort_by_g(E* ev, uint32_t k) {
     }
 }
 
-SNN_HD uint32_t f2u(float f) {
-    union { float f; uint32_t u; } c;
-    c.f = f;
-    return c.u;
-}
-SNN_HD float u2f(uint32_t u) {
-    union { float f; uint32_t u; } c;
-    c.u = u;
-    return c.f;
-}
-
 // Bring a record up to `fires` post-synaptic updates of tick T (apply_post_spike_plasticity,
 // SynapseDendriteProcessor.cpp:138-173, once per fire of the target at T).
 SNN_HD StdpTally post_catch_up(const Step& S, StdpRec& r, uint32_t T, uint32_t fires) {
@@ -342,7 +355,9 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
                 // first the post-synaptic updates of this tick's earlier fires of x
                 StdpTally changed = post_catch_up(S, r, T, fires);
                 // pre-synaptic side: process_synaptic_input (:95-126); the code offset is ignored
-                const uint32_t dl = CODE_HERE ? e.slot : S.delay[e.slot];
+                uint32_t dl;
+                if constexpr (CODE_HERE) dl = f2u(e.w);
+                else dl = S.delay[e.slot];
                 const uint32_t pre = T >= dl ? T - dl : T;
                 if (r.last_post != kNone) changed.add(stdp_apply(r, pre, r.last_post, S.ltp, S.ltd, S.wmax));
                 r.last_pre = pre;
@@ -350,11 +365,13 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
                 S.stdp[e.tpos] = r;
                 w = r.weight;
             }
-            if (!CODE_HERE && e.slot != kNone) {
-                const uint32_t slot = e.slot;
-                if (CODE2) err |= code_observe2(S.code2[slot], T, 1, p.window, p.sim_thr, p.max_refs);
-                else err |= code_observeN(S.codeN[slot], T, 1, code_expected(S, slot), p.window, p.sim_thr, p.max_refs);
-                if (S.pre_count) S.pre_count[slot] += 1;
+            if constexpr (!CODE_HERE) {
+                if (e.slot != kNone) {
+                    const uint32_t slot = e.slot;
+                    if (CODE2) err |= code_observe2(S.code2[slot], T, 1, p.window, p.sim_thr, p.max_refs);
+                    else err |= code_observeN(S.codeN[slot], T, 1, code_expected(S, slot), p.window, p.sim_thr, p.max_refs);
+                    if (S.pre_count) S.pre_count[slot] += 1;
+                }
             }
         }
         // process_input (:175-212)

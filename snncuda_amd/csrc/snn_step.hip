@@ -386,7 +386,7 @@ constexpr uint32_t kScatterEpt = 2;  // events per thread per round
 // at most 64 VGPRs: two 1024-thread workgroups per CU (8 waves per SIMD)
 template <bool WIDE, bool CODE_HERE, bool CODE2>
 __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8))) k_pscatter(Step S) {
-    typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
+    typedef typename EntryOf<WIDE, CODE_HERE>::type E;
     extern __shared__ uint32_t dyn[];
     __shared__ SegCache cache;
     const uint32_t np = S.nparts, ch = S.chunk;
@@ -456,27 +456,27 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
     if (err) atomicOr(&S.ctl[1], err);
 }
 
-// LDS stage of one fold round; with the 4096-neuron partition counters (32 KB) and the
-// touched-target list (8 KB) the fold uses all but 3 KB of the 160 KB of LDS a workgroup may hold
-constexpr uint32_t kStageBytes = 114688;
+// LDS stage of one fold round; with the 4096-neuron partition's places (16 KB) and the per-target
+// cursors (16 KB) the fold uses all but 2 KB of the 160 KB of LDS a workgroup may hold
+constexpr uint32_t kStageBytes = 129024;
 constexpr uint32_t kMaxPartition = 4096;  // neurons per partition at most (partition shift <= 12)
 constexpr uint32_t kRound = kPBlock;  // targets folded per round (one per thread)
 
-// One workgroup per target partition. The partition's targets are folded in rounds of 1024 (one
-// per thread): a counting sort by target over the whole partition gives every target's place, and
-// each round stages only its own targets' events in LDS (global scratch if they do not fit), so
-// the bucket's events are read once and its 2-byte target indices once plus once per round.
+// One workgroup per target partition. A counting sort by target over the partition gives every
+// target's place. Dense partitions are folded in rounds of 1024 targets (one per thread), each round
+// staging only its own targets' events in LDS (global scratch if they do not fit), so the bucket's
+// 2-byte target indices are read once plus once per round and each event once. Partitions whose
+// events all fit the stage take one round over their touched targets only.
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
 __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
-    typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
+    typedef typename EntryOf<WIDE, CODE_HERE>::type E;
     constexpr uint32_t kCap = kStageBytes / sizeof(E);
     __shared__ E lds_stage[kCap];
-    __shared__ uint32_t fill[kRound];
-    __shared__ uint16_t touched[kMaxPartition];  // sparse partitions: the targets with events
-    extern __shared__ uint32_t sh[];
+    // dense rounds: each target's next stage slot (1024 entries); sparse partitions: per touched
+    // target (by rank) its index | next stage slot << 16
+    __shared__ uint32_t aux[kMaxPartition];
+    extern __shared__ uint32_t off[];  // PS + 1: event counts per target, then their places
     const uint32_t PS = 1u << S.pshift;
-    uint32_t* cnt = sh;
-    uint32_t* off = sh + PS;  // PS + 1 entries
     const uint32_t p = blockIdx.x;
     const uint32_t T = S.ctl[0];
     const uint32_t start = S.hist[(uint64_t)p * S.nwg];
@@ -484,11 +484,11 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
     const uint32_t c = end - start;
     const E* src = (const E*)S.prec + start;
     const uint16_t* tls = S.ptl + start;
-    for (uint32_t t = threadIdx.x; t < PS; t += blockDim.x) cnt[t] = 0;
+    for (uint32_t t = threadIdx.x; t < PS; t += blockDim.x) off[t] = 0;
     __syncthreads();
     // the passes over the bucket load kUnroll records per thread before using them, so that many
     // loads are in flight per wave
-    constexpr uint32_t kUnroll = 8;
+    constexpr uint32_t kUnroll = WIDE ? 4 : 8;
     constexpr uint32_t kNoTarget = 0xFFFFu;
     for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * kPBlock) {
         uint32_t tv[kUnroll];
@@ -496,7 +496,7 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
         for (uint32_t u = 0; u < kUnroll; ++u) tv[u] = i0 + u * kPBlock < c ? tls[i0 + u * kPBlock] : kNoTarget;
 #pragma unroll
         for (uint32_t u = 0; u < kUnroll; ++u)
-            if (tv[u] != kNoTarget) atomicAdd(&cnt[tv[u]], 1u);
+            if (tv[u] != kNoTarget) atomicAdd(&off[tv[u]], 1u);
     }
     __syncthreads();
     uint32_t err = 0;
@@ -507,15 +507,14 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
         uint32_t run = 0;
         for (uint32_t t0 = 0; t0 < PS; t0 += blockDim.x) {
             const uint32_t t = t0 + threadIdx.x;
-            const uint32_t v = cnt[t];
+            const uint32_t v = off[t];
             uint32_t tot;
             const uint32_t ex = run + block_excl_scan_n<kPBlock>(v | (v ? 1u << 16 : 0u), &tot);
-            off[t] = ex & 0xFFFFu;
-            cnt[t] = ex & 0xFFFFu;  // fill cursor
-            if (v) touched[ex >> 16] = (uint16_t)t;
+            off[t] = ex;
+            if (v) aux[ex >> 16] = t | (ex << 16);
             run += tot;
         }
-        if (threadIdx.x == 0) off[PS] = run & 0xFFFFu;
+        if (threadIdx.x == 0) off[PS] = run;
         const uint32_t ntouched = run >> 16;
         __syncthreads();
         for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * kPBlock) {
@@ -529,20 +528,21 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
             }
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u)
-                if (tv[u] != kNoTarget) lds_stage[atomicAdd(&cnt[tv[u]], 1u)] = ev[u];
+                if (tv[u] != kNoTarget) lds_stage[atomicAdd(&aux[off[tv[u]] >> 16], 1u << 16) >> 16] = ev[u];
         }
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < ntouched; j += blockDim.x) {
-            const uint32_t t = touched[j];
+            const uint32_t t = aux[j] & 0xFFFFu;
             const uint32_t x = (p << S.pshift) + t;
-            if (x < S.n) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2>(S, lds_stage + off[t], off[t + 1] - off[t], x, T);
+            const uint32_t at = off[t] & 0xFFFFu;
+            if (x < S.n) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2>(S, lds_stage + at, (off[t + 1] & 0xFFFFu) - at, x, T);
         }
         if (err) atomicOr(&S.ctl[1], err);
         return;
     }
     uint32_t run = 0;
     for (uint32_t t0 = 0; t0 < PS; t0 += blockDim.x) {
-        const uint32_t v = cnt[t0 + threadIdx.x];
+        const uint32_t v = off[t0 + threadIdx.x];
         uint32_t tot;
         const uint32_t ex = block_excl_scan_n<kPBlock>(v, &tot);
         off[t0 + threadIdx.x] = run + ex;
@@ -550,6 +550,7 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
     }
     if (threadIdx.x == 0) off[PS] = run;
     __syncthreads();
+    uint32_t* fill = aux;
     const uint32_t rounds = PS / kRound;
     for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t lo = off[r * kRound], hi = off[(r + 1) * kRound];
@@ -571,7 +572,8 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
         }
         __syncthreads();
         const uint32_t x = (p << S.pshift) + t;
-        if (x < S.n && cnt[t]) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2>(S, st + (off[t] - lo), cnt[t], x, T);
+        const uint32_t k = off[t + 1] - off[t];
+        if (x < S.n && k) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2>(S, st + (off[t] - lo), k, x, T);
         __syncthreads();
     }
     if (err) atomicOr(&S.ctl[1], err);
@@ -1162,7 +1164,7 @@ constexpr uint32_t kScatterDynLds = 64 * 1024;
 
 template <bool WIDE, bool CODE_HERE, bool CODE2>
 void launch_scatter(snn_net* net) {
-    typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
+    typedef typename EntryOf<WIDE, CODE_HERE>::type E;
     Step& S = net->S;
     const uint32_t want = WIDE ? 8u : 16u;  // 128 bytes of events per flush
     const uint64_t room = kScatterDynLds > 8ull * S.nparts ? kScatterDynLds - 8ull * S.nparts : 0;
@@ -1177,7 +1179,7 @@ void launch_scatter(snn_net* net) {
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
 void launch_fold(snn_net* net) {
     hipLaunchKernelGGL((k_pfold<WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kPBlock),
-                       (2u * (1u << net->S.pshift) + 1) * 4, net->stream, net->S);
+                       ((1u << net->S.pshift) + 1) * 4, net->stream, net->S);
 }
 
 // The per-slot compact code records: the network's own, or with per-source code patterns a

@@ -74,7 +74,7 @@ void scan_excl(const std::vector<uint32_t>& in, uint32_t n, std::vector<uint32_t
 // events of the tick in a shuffled order, grouped by target, then folded target by target
 template <bool WIDE, bool ST, bool CH, bool C2>
 uint32_t deliver(Emu& m, const std::vector<std::pair<uint32_t, uint32_t>>& order, uint32_t T) {
-    typedef typename std::conditional<WIDE, Entry16, Entry8>::type E;
+    typedef typename EntryOf<WIDE, CH>::type E;
     std::vector<uint32_t> xs(order.size());
     std::vector<E> recs(order.size());
     uint32_t err = 0;
