@@ -48,7 +48,10 @@ SCATTER_BYTES_PER_EVENT = 9.0 + 10.0
 # candidate's last run (4 B read, 4 B written)
 FOLD_BYTES_PER_EVENT = 10.0
 FOLD_BYTES_PER_NEURON = 16.0 + 2 * 32.0 + 2 * 32.0 + 8.0
-# STDP on adds 12 B per event (pre-side record) and 20 B per (fire, incoming plastic synapse)
+# with STDP the bucket record is 12 + 2 B and a plastic event reads and writes its 32-B STDP record
+FOLD_STDP_BYTES_PER_EVENT = 14.0
+FOLD_STDP_RECORD_BYTES = 2 * 32.0
+# STDP on adds 12 B per plastic event (pre-side record) and 20 B per (fire, incoming plastic synapse)
 STDP_EVENT_BYTES = 12.0
 STDP_POST_BYTES = 20.0
 
@@ -358,16 +361,31 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(traffic_file),
                 "bytes_per_launch": bytes_per_launch, "mean_launch_ms": avg_s * 1e3}
 
+    # plastic share of the synapses (excitatory and input neurons' synapses; targets uniform, so it
+    # is also the share of the delivered events and the mean plastic in-degree over n)
+    fan = np.asarray(spec["fanout"], np.int64)
+    plastic_syn = int(fan[:spec["n_exc"]].sum()) + spec["n_sources"] * spec["source_fanout"] if args.stdp else 0
+    pfrac = plastic_syn / E
+    if args.stdp:
+        fold_per_event = FOLD_STDP_BYTES_PER_EVENT + FOLD_STDP_RECORD_BYTES * pfrac
+    else:
+        fold_per_event = FOLD_BYTES_PER_EVENT
     fold_rl = kernel_roofline("fold", "k_pfold (partition fold)",
-                              FOLD_BYTES_PER_EVENT * ev_per_launch + FOLD_BYTES_PER_NEURON * touched,
+                              fold_per_event * ev_per_launch + FOLD_BYTES_PER_NEURON * touched,
                               "fold_traffic.json")
-    fold_rl["bytes_model"] = {"per_event": FOLD_BYTES_PER_EVENT, "per_touched_neuron": FOLD_BYTES_PER_NEURON,
-                              "touched_neurons": touched}
+    fold_rl["bytes_model"] = {"per_event": fold_per_event, "per_touched_neuron": FOLD_BYTES_PER_NEURON,
+                              "touched_neurons": touched, "plastic_fraction": pfrac}
     scat_rl = kernel_roofline("scatter", "k_pscatter (partition scatter)", SCATTER_BYTES_PER_EVENT * ev_per_launch,
                               "scatter_traffic.json")
     scat_rl["bytes_per_event"] = SCATTER_BYTES_PER_EVENT
     ms_per_step = elapsed / args.steps * 1e3
-    step_gbs = STEP_BYTES_PER_EVENT * events / elapsed / 1e9
+    # the survey's step model; with STDP the pre-side bytes per plastic event and the post-side bytes
+    # per (fire, incoming plastic synapse), fires x mean plastic in-degree
+    step_bytes = STEP_BYTES_PER_EVENT * events
+    if args.stdp:
+        fires = float(fpt.mean()) * args.steps
+        step_bytes += STDP_EVENT_BYTES * pfrac * events + STDP_POST_BYTES * fires * plastic_syn / args.neurons
+    step_gbs = step_bytes / elapsed / 1e9
     line = {
         "metric": "synaptic events/s (1M-neuron net, % HBM roofline)",
         "value": events / elapsed,
@@ -397,7 +415,7 @@ def main():
         "scatter_roofline": scat_rl,
         "step_roofline": {
             "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
-            "bytes_per_event": STEP_BYTES_PER_EVENT,
+            "bytes_per_event": step_bytes / max(events, 1),
         },
         "kernel_ms_per_tick": {k: v[0] / max(v[1], 1) for k, v in kt.items() if v[1]},
         "schedule": schedule,
