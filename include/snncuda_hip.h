@@ -150,7 +150,8 @@ typedef struct snn_schedule {
     uint32_t partition_shift;
     uint32_t scatter_workgroups;
     uint32_t graph_ticks;
-    uint32_t reserved;
+    uint32_t fold_threads;      /* fold workgroup: 1024, or 256 (small LDS stage, more workgroups per
+                                   CU: faster when few events reach each partition); 0 on set: 1024 */
 } snn_schedule;
 int snn_net_get_schedule(snn_net* net, snn_schedule* out);
 int snn_net_set_schedule(snn_net* net, const snn_schedule* schedule);

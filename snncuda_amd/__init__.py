@@ -70,7 +70,7 @@ class _EiDesc(ctypes.Structure):
 
 class Schedule(ctypes.Structure):
     _fields_ = [("partition_shift", ctypes.c_uint32), ("scatter_workgroups", ctypes.c_uint32),
-                ("graph_ticks", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("graph_ticks", ctypes.c_uint32), ("fold_threads", ctypes.c_uint32)]
 
 
 def library_path() -> pathlib.Path:
@@ -338,12 +338,12 @@ class Network:
         sc = Schedule()
         _check(lib().snn_net_get_schedule(self._h, ctypes.byref(sc)))
         return dict(partition_shift=sc.partition_shift, scatter_workgroups=sc.scatter_workgroups,
-                    graph_ticks=sc.graph_ticks)
+                    graph_ticks=sc.graph_ticks, fold_threads=sc.fold_threads)
 
     def set_schedule(self, **kw) -> None:
         cur = self.schedule()
         cur.update({k: v for k, v in kw.items() if v is not None})
-        sc = Schedule(cur["partition_shift"], cur["scatter_workgroups"], cur["graph_ticks"], 0)
+        sc = Schedule(cur["partition_shift"], cur["scatter_workgroups"], cur["graph_ticks"], cur["fold_threads"])
         _check(lib().snn_net_set_schedule(self._h, ctypes.byref(sc)))
 
     def autotune(self, trial_ticks: int = 32) -> None:

@@ -460,21 +460,24 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
 // cursors (16 KB) the fold uses all but 2 KB of the 160 KB of LDS a workgroup may hold
 constexpr uint32_t kStageBytes = 129024;
 constexpr uint32_t kMaxPartition = 4096;  // neurons per partition at most (partition shift <= 12)
-constexpr uint32_t kRound = kPBlock;  // targets folded per round (one per thread)
+constexpr int kSmallFold = 256;              // the small fold workgroup (sparse workloads)
+constexpr uint32_t kSmallStageBytes = 16384;  // its LDS stage: four workgroups per CU
 
 // One workgroup per target partition. A counting sort by target over the partition gives every
 // target's place. Dense partitions are folded in rounds of 1024 targets (one per thread), each round
 // staging only its own targets' events in LDS (global scratch if they do not fit), so the bucket's
 // 2-byte target indices are read once plus once per round and each event once. Partitions whose
 // events all fit the stage take one round over their touched targets only.
-template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
-__global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
+template <int BLOCK, bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
+__global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
     typedef typename EntryOf<WIDE, CODE_HERE>::type E;
-    constexpr uint32_t kCap = kStageBytes / sizeof(E);
+    constexpr uint32_t kRound = BLOCK;  // targets folded per round (one per thread)
+    constexpr uint32_t kCap = (BLOCK == kPBlock ? kStageBytes : kSmallStageBytes) / sizeof(E);
     __shared__ E lds_stage[kCap];
-    // dense rounds: each target's next stage slot (1024 entries); sparse partitions: per touched
-    // target (by rank) its index | next stage slot << 16
-    __shared__ uint32_t aux[kMaxPartition];
+    // dense rounds: each target's next stage slot (kRound entries); sparse partitions: per touched
+    // target (by rank, at most kCap) its index | next stage slot << 16
+    constexpr uint32_t kAux = BLOCK == kPBlock ? kMaxPartition : (kCap > kRound ? kCap : kRound);
+    __shared__ uint32_t aux[kAux];
     extern __shared__ uint32_t off[];  // PS + 1: event counts per target, then their places
     const uint32_t PS = 1u << S.pshift;
     const uint32_t p = blockIdx.x;
@@ -490,10 +493,10 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
     // loads are in flight per wave
     constexpr uint32_t kUnroll = WIDE ? 4 : 8;
     constexpr uint32_t kNoTarget = 0xFFFFu;
-    for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * kPBlock) {
+    for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * BLOCK) {
         uint32_t tv[kUnroll];
 #pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) tv[u] = i0 + u * kPBlock < c ? tls[i0 + u * kPBlock] : kNoTarget;
+        for (uint32_t u = 0; u < kUnroll; ++u) tv[u] = i0 + u * BLOCK < c ? tls[i0 + u * BLOCK] : kNoTarget;
 #pragma unroll
         for (uint32_t u = 0; u < kUnroll; ++u)
             if (tv[u] != kNoTarget) atomicAdd(&off[tv[u]], 1u);
@@ -509,7 +512,7 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
             const uint32_t t = t0 + threadIdx.x;
             const uint32_t v = off[t];
             uint32_t tot;
-            const uint32_t ex = run + block_excl_scan_n<kPBlock>(v | (v ? 1u << 16 : 0u), &tot);
+            const uint32_t ex = run + block_excl_scan_n<BLOCK>(v | (v ? 1u << 16 : 0u), &tot);
             off[t] = ex;
             if (v) aux[ex >> 16] = t | (ex << 16);
             run += tot;
@@ -517,12 +520,12 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
         if (threadIdx.x == 0) off[PS] = run;
         const uint32_t ntouched = run >> 16;
         __syncthreads();
-        for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * kPBlock) {
+        for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * BLOCK) {
             uint32_t tv[kUnroll];
             E ev[kUnroll];
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u) {
-                const uint32_t i = i0 + u * kPBlock;
+                const uint32_t i = i0 + u * BLOCK;
                 tv[u] = i < c ? tls[i] : kNoTarget;
                 if (i < c) ev[u] = src[i];
             }
@@ -544,7 +547,7 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
     for (uint32_t t0 = 0; t0 < PS; t0 += blockDim.x) {
         const uint32_t v = off[t0 + threadIdx.x];
         uint32_t tot;
-        const uint32_t ex = block_excl_scan_n<kPBlock>(v, &tot);
+        const uint32_t ex = block_excl_scan_n<BLOCK>(v, &tot);
         off[t0 + threadIdx.x] = run + ex;
         run += tot;
     }
@@ -558,14 +561,14 @@ __global__ void __launch_bounds__(kPBlock) k_pfold(Step S) {
         fill[threadIdx.x] = off[t] - lo;
         __syncthreads();
         E* st = hi - lo <= kCap ? lds_stage : (E*)S.stage + start + lo;
-        for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * kPBlock) {
+        for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * BLOCK) {
             uint32_t tv[kUnroll];
             E ev[kUnroll];
 #pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) tv[u] = i0 + u * kPBlock < c ? tls[i0 + u * kPBlock] : kNoTarget;
+            for (uint32_t u = 0; u < kUnroll; ++u) tv[u] = i0 + u * BLOCK < c ? tls[i0 + u * BLOCK] : kNoTarget;
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u)
-                if (tv[u] / kRound == r) ev[u] = src[i0 + u * kPBlock];
+                if (tv[u] / kRound == r) ev[u] = src[i0 + u * BLOCK];
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u)
                 if (tv[u] / kRound == r) st[atomicAdd(&fill[tv[u] % kRound], 1u)] = ev[u];
@@ -1098,6 +1101,7 @@ struct snn_net {
     uint64_t e = 0;
     bool wide = false, stdp = false, code_here = true, code2 = true;
     uint32_t grid = 1024;
+    uint32_t fold_threads = 1024;  // the fold variant (Schedule::fold)
     uint32_t fold_grid = 1;
     uint32_t* partials = nullptr;
     // host-side run state
@@ -1178,8 +1182,13 @@ void launch_scatter(snn_net* net) {
 
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
 void launch_fold(snn_net* net) {
-    hipLaunchKernelGGL((k_pfold<WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kPBlock),
-                       ((1u << net->S.pshift) + 1) * 4, net->stream, net->S);
+    const size_t lds = ((1u << net->S.pshift) + 1) * 4;
+    if (net->fold_threads == (uint32_t)kSmallFold)
+        hipLaunchKernelGGL((k_pfold<kSmallFold, WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kSmallFold),
+                           lds, net->stream, net->S);
+    else
+        hipLaunchKernelGGL((k_pfold<kPBlock, WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kPBlock), lds,
+                           net->stream, net->S);
 }
 
 // The per-slot compact code records: the network's own, or with per-source code patterns a
@@ -1344,9 +1353,10 @@ int guarded(F&& f) {
 
 struct Schedule {
     uint32_t pshift, nwg, graph;
+    uint32_t fold;  // fold workgroup threads: 1024, or 256 (more workgroups per CU, small LDS stage)
 };
 
-Schedule current_schedule(const snn_net* net) { return {net->S.pshift, net->S.nwg, net->graph_ticks}; }
+Schedule current_schedule(const snn_net* net) { return {net->S.pshift, net->S.nwg, net->graph_ticks, net->fold_threads}; }
 
 void apply_schedule(snn_net* net, const Schedule& sc) {
     Step& S = net->S;
@@ -1355,10 +1365,13 @@ void apply_schedule(snn_net* net, const Schedule& sc) {
     if (parts > net->hist_cap_parts || parts * 4 > 65536) throw std::invalid_argument("partition_shift too small");
     if (sc.nwg < 1 || sc.nwg > net->max_nwg) throw std::invalid_argument("scatter_workgroups out of range");
     if (sc.graph > 1024) throw std::invalid_argument("graph_ticks above 1024");
+    if (sc.fold != (uint32_t)kPBlock && sc.fold != (uint32_t)kSmallFold)
+        throw std::invalid_argument("fold_threads must be 1024 or 256");
     S.pshift = sc.pshift;
     S.nparts = (uint32_t)parts;
     S.nwg = sc.nwg;
     net->graph_ticks = sc.graph;
+    net->fold_threads = sc.fold;
 }
 
 // The launches of graph_ticks consecutive ticks captured as one HIP graph. Kernel arguments are
@@ -1378,7 +1391,7 @@ struct TickGraph {
 
 void ensure_graph(snn_net* net, uint32_t t0, TickGraph& g) {
     const Schedule sc = current_schedule(net);
-    if (g.exec && g.sc.pshift == sc.pshift && g.sc.nwg == sc.nwg && g.sc.graph == sc.graph) return;
+    if (g.exec && g.sc.pshift == sc.pshift && g.sc.nwg == sc.nwg && g.sc.graph == sc.graph && g.sc.fold == sc.fold) return;
     g.reset();
     Timer off{net};
     off.enabled = false;
@@ -1424,6 +1437,12 @@ uint64_t autotune(snn_net* net, uint32_t t0, uint64_t nt, Timer& tm, TickGraph& 
         }
         return c;
     };
+    auto fold_cands = [&](Schedule b) {
+        std::vector<Schedule> c{b, b};
+        c[0].fold = (uint32_t)kPBlock;
+        c[1].fold = (uint32_t)kSmallFold;
+        return c;
+    };
     auto part_cands = [&](Schedule b) {
         std::vector<Schedule> c;
         for (uint32_t ps = 10; ps <= 12; ++ps) {
@@ -1436,7 +1455,8 @@ uint64_t autotune(snn_net* net, uint32_t t0, uint64_t nt, Timer& tm, TickGraph& 
         return c;
     };
     const Schedule start = current_schedule(net);
-    const uint64_t need = 2ull * trial * (graph_cands(start).size() + nwg_cands(start).size() + part_cands(start).size());
+    const uint64_t need = 2ull * trial *
+                          (graph_cands(start).size() + nwg_cands(start).size() + part_cands(start).size() + 2);
     if (trial == 0 || nt < need + trial) return 0;
     hipEvent_t a = nullptr, b = nullptr;
     check(hipEventCreate(&a), "hipEventCreate");
@@ -1446,8 +1466,11 @@ uint64_t autotune(snn_net* net, uint32_t t0, uint64_t nt, Timer& tm, TickGraph& 
     Schedule best = start;
     std::string log;
     try {
-        for (int dim = 0; dim < 3; ++dim) {
-            const std::vector<Schedule> cands = dim == 0 ? graph_cands(best) : dim == 1 ? nwg_cands(best) : part_cands(best);
+        for (int dim = 0; dim < 4; ++dim) {
+            const std::vector<Schedule> cands = dim == 0   ? graph_cands(best)
+                                                : dim == 1 ? nwg_cands(best)
+                                                : dim == 2 ? part_cands(best)
+                                                           : fold_cands(best);
             std::vector<float> t(cands.size(), 1e30f);
             for (int round = 0; round < 2; ++round)
                 for (size_t c = 0; c < cands.size(); ++c) {
@@ -1474,7 +1497,8 @@ uint64_t autotune(snn_net* net, uint32_t t0, uint64_t nt, Timer& tm, TickGraph& 
             best = cands[k];
             for (size_t c = 0; c < cands.size(); ++c)
                 log += "shift=" + std::to_string(cands[c].pshift) + " wgs=" + std::to_string(cands[c].nwg) +
-                       " graph=" + std::to_string(cands[c].graph) + " score=" + std::to_string(t[c]) +
+                       " graph=" + std::to_string(cands[c].graph) + " fold=" + std::to_string(cands[c].fold) +
+                       " score=" + std::to_string(t[c]) +
                        (c == k ? " *" : "") + "\n";
         }
     } catch (...) {
@@ -1979,14 +2003,15 @@ int snn_net_get_schedule(snn_net* net, snn_schedule* out) {
     out->partition_shift = net->S.pshift;
     out->scatter_workgroups = net->S.nwg;
     out->graph_ticks = net->graph_ticks;
-    out->reserved = 0;
+    out->fold_threads = net->fold_threads;
     return SNN_OK;
 }
 
 int snn_net_set_schedule(snn_net* net, const snn_schedule* sc) {
     if (!net || !sc) return fail(SNN_EINVAL, "null argument");
     return guarded([&]() -> int {
-        apply_schedule(net, Schedule{sc->partition_shift, sc->scatter_workgroups, sc->graph_ticks});
+        apply_schedule(net, Schedule{sc->partition_shift, sc->scatter_workgroups, sc->graph_ticks,
+                                     sc->fold_threads ? sc->fold_threads : (uint32_t)kPBlock});
         return SNN_OK;
     });
 }

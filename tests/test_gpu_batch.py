@@ -67,7 +67,8 @@ def test_schedules_and_autotune_leave_results_unchanged():
         ref = _snapshot(b)
         base = b.union.schedule()
     for sc in (dict(graph_ticks=16), dict(partition_shift=12, scatter_workgroups=64),
-               dict(partition_shift=10, scatter_workgroups=1024, graph_ticks=7)):
+               dict(partition_shift=10, scatter_workgroups=1024, graph_ticks=7), dict(fold_threads=256),
+               dict(partition_shift=11, fold_threads=256, graph_ticks=5)):
         print(f"schedule {sc}", flush=True)
         with _run_batch(nets, schedule=sc) as b:
             got = b.union.schedule()

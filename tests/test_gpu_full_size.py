@@ -1,9 +1,10 @@
 """The 1M-neuron workload at full size (about 2.07e9 synapses, about 1e7 delivered events per tick)
 on the MI355X, through size-independent properties: the oracle cannot run it, but the results must
 not depend on how the step is scheduled. Target partitions of 4,096 and 2,048 neurons, different
-scatter workgroup counts and HIP graph replay change every kernel's work split (bucket runs, fold
-rounds, the dense fold path and its global-scratch fallback), so the rasters, counters and every
-neuron's state must be bit-identical across them; the per-tick counters must add up to the totals.
+scatter workgroup counts, the two fold workgroup sizes and HIP graph replay change every kernel's
+work split (bucket runs, fold rounds, the dense fold path and its global-scratch fallback), so the
+rasters, counters and every neuron's state must be bit-identical across them; the per-tick counters
+must add up to the totals.
 STDP on uses the bench's regime (excitatory synapses plastic, threshold 2.75)."""
 import numpy as np
 import pytest
@@ -26,7 +27,8 @@ def _run(spec, injects, schedule):
     with sa.Network(spec=spec, max_events_per_tick=1 << 27, fire_log_capacity=1 << 26, raster_capacity=1 << 24,
                     track_pre_count=False) as dev:
         dev.run_commands(injects)
-        dev.set_schedule(partition_shift=schedule[0], scatter_workgroups=schedule[1], graph_ticks=schedule[2])
+        dev.set_schedule(partition_shift=schedule[0], scatter_workgroups=schedule[1], graph_ticks=schedule[2],
+                         fold_threads=schedule[3])
         dev.run_until(TICKS - 1)
         nrn = dev.neurons()
         d, f = dev.tick_counts(0, TICKS)
@@ -39,14 +41,17 @@ def test_full_size_results_do_not_depend_on_the_schedule(stdp):
     spec, cmds = W.config2_spec(0, n=1_000_000, ticks=TICKS, threshold=2.75 if stdp else 2.5, w_inh=5.0,
                                 kick=0.02, plastic="exc" if stdp else False)
     injects = [c for c in cmds if c[0] == 0]
-    a = _run(spec, injects, (12, 512, 0))
+    a = _run(spec, injects, (12, 512, 0, 1024))
     print(f"  stdp={stdp}: {a['delivered']} delivered, {a['fired']} fired in {TICKS} ticks", flush=True)
     # the regime is the dense one the bench measures: millions of events per tick
     assert a["per_tick"][0][TICKS // 2:].mean() > 1e6
     assert int(a["per_tick"][0].sum()) == a["delivered"] and int(a["per_tick"][1].sum()) == a["fired"]
     assert a["raster"].shape[0] == a["fired"] and int(a["spikes"].sum()) == a["fired"]
-    b = _run(spec, injects, (11, 256, 16))
-    assert a["delivered"] == b["delivered"] and a["fired"] == b["fired"]
-    assert np.array_equal(a["raster"], b["raster"])
-    for k in ("membrane", "current", "spikes"):
-        assert np.array_equal(a[k], b[k]), k
+    # the 256-thread fold stages rounds of 256 targets in a 16 KB stage: at this density most rounds
+    # take its global-scratch path
+    for sc in ((11, 256, 16, 1024), (12, 512, 0, 256)):
+        b = _run(spec, injects, sc)
+        assert a["delivered"] == b["delivered"] and a["fired"] == b["fired"], sc
+        assert np.array_equal(a["raster"], b["raster"]), sc
+        for k in ("membrane", "current", "spikes"):
+            assert np.array_equal(a[k], b[k]), (sc, k)
