@@ -36,7 +36,9 @@ namespace snn {
 
 constexpr uint64_t kInjFlag = 1ull << 63;
 constexpr uint32_t kGroupInj = 0x80000000u;
-constexpr uint32_t kMaxEventsPerTick = 1u << 28;  // g is packed in 28 bits
+// g is packed in 28 bits; with g < 2^28 - 1 no entry's gk is all ones, the value that pads the
+// per-target sorting networks
+constexpr uint32_t kMaxEventsPerTick = (1u << 28) - 1;
 
 SNN_HD uint32_t f2u(float f) {
     union { float f; uint32_t u; } c;
@@ -294,10 +296,104 @@ SNN_HD uint32_t make_event(const Step& S, uint64_t v0, uint32_t g0, uint32_t mul
     return err;
 }
 
+// Batcher's odd-even merge sort network for the next power of two >= M, keeping only the
+// compare-exchanges within the first M places: padded with maximal keys, places M.. hold the padding
+// through every stage, so a compare-exchange that reaches them never swaps.
+constexpr uint32_t kMaxNetCx = 192;
+struct SortNet {
+    uint32_t n;
+    uint8_t lo[kMaxNetCx], hi[kMaxNetCx];
+};
+constexpr SortNet odd_even_net(uint32_t M) {
+    uint32_t N = 1;
+    while (N < M) N <<= 1;
+    SortNet s{};
+    for (uint32_t p = 1; p < N; p <<= 1)
+        for (uint32_t k = p; k >= 1; k >>= 1)
+            for (uint32_t j = k % p; j + k < N; j += 2 * k)
+                for (uint32_t i = 0; i < k && i + j + k < N; ++i)
+                    if ((i + j) / (2 * p) == (i + j + k) / (2 * p) && i + j + k < M) {
+                        s.lo[s.n] = (uint8_t)(i + j);
+                        s.hi[s.n] = (uint8_t)(i + j + k);
+                        ++s.n;
+                    }
+    return s;
+}
+
+// k <= M entries sorted by g in registers: no dependent LDS round trips and no divergence inside
+// the network (g is unique within a tick, so comparing whole gk words compares g)
+template <uint32_t M, typename E>
+SNN_HD void net_sort_by_g(E* ev, uint32_t k) {
+    constexpr SortNet net = odd_even_net(M);
+    static_assert(net.n <= kMaxNetCx, "sorting network table too small");
+    E a[M];
+#pragma unroll
+    for (uint32_t j = 0; j < M; ++j) {
+        if (j < k) {
+            a[j] = ev[j];
+        } else {
+            a[j] = E{};
+            a[j].gk = 0xFFFFFFFFu;
+        }
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < net.n; ++c) {
+        E& x = a[net.lo[c]];
+        E& y = a[net.hi[c]];
+        const bool sw = x.gk > y.gk;
+        const E lo = sw ? y : x;
+        const E hi = sw ? x : y;
+        x = lo;
+        y = hi;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < M; ++j)
+        if (j < k) ev[j] = a[j];
+}
+
+// The same with one 32-bit key per entry, g << 5 | its place (g < 2^27): a compare-exchange is a
+// min and a max, and the entries are gathered by place afterwards.
+template <uint32_t M, typename E>
+SNN_HD void key_sort_by_g(E* ev, uint32_t k) {
+    static_assert(M <= 32, "the place takes 5 bits of the key");
+    constexpr SortNet net = odd_even_net(M);
+    static_assert(net.n <= kMaxNetCx, "sorting network table too small");
+    uint32_t key[M];
+#pragma unroll
+    for (uint32_t j = 0; j < M; ++j) key[j] = j < k ? ((ev[j].gk >> 4) << 5) | j : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t c = 0; c < net.n; ++c) {
+        const uint32_t x = key[net.lo[c]], y = key[net.hi[c]];
+        key[net.lo[c]] = x < y ? x : y;
+        key[net.hi[c]] = x < y ? y : x;
+    }
+    E a[M];
+#pragma unroll
+    for (uint32_t j = 0; j < M; ++j)
+        if (j < k) a[j] = ev[key[j] & 31u];
+#pragma unroll
+    for (uint32_t j = 0; j < M; ++j)
+        if (j < k) ev[j] = a[j];
+}
+
+// narrow_g: every g of the tick is below 2^27
 template <typename E>
-SNN_HD void sort_by_g(E* ev, uint32_t k) {
-    // Shell sort (Ciura gaps) in place; k is small for almost every target. The gap loop is
-    // unrolled so the gap table stays in registers.
+SNN_HD void sort_by_g(E* ev, uint32_t k, bool narrow_g) {
+    if (k <= 1) return;
+    if constexpr (sizeof(E) == sizeof(Entry8)) {
+        if (k <= 24) {
+            if (narrow_g) {
+                if (k <= 8) key_sort_by_g<8>(ev, k);
+                else key_sort_by_g<24>(ev, k);
+            } else {
+                if (k <= 8) net_sort_by_g<8>(ev, k);
+                else net_sort_by_g<24>(ev, k);
+            }
+            return;
+        }
+    }
+    // Shell sort (Ciura gaps) in place for the rest. The gap loop is unrolled so the gap table stays
+    // in registers.
 #pragma unroll
     for (uint32_t gi = 0; gi < 8; ++gi) {
         const uint32_t gap = gi == 0 ? 701u : gi == 1 ? 301u : gi == 2 ? 132u : gi == 3 ? 57u
@@ -334,7 +430,7 @@ SNN_HD StdpTally post_catch_up(const Step& S, StdpRec& r, uint32_t T, uint32_t f
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2, typename E>
 SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32_t T) {
     if (k == 0) return 0;
-    sort_by_g(ev, k);
+    sort_by_g(ev, k, S.seg_count[1] <= (1u << 27));
 
     const NeuronParams p = S.np[x];
     NeuronDyn d = S.dyn[x];

@@ -555,12 +555,10 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
     __syncthreads();
     uint32_t* fill = aux;
     const uint32_t rounds = PS / kRound;
-    for (uint32_t r = 0; r < rounds; ++r) {
-        const uint32_t lo = off[r * kRound], hi = off[(r + 1) * kRound];
+    // one round staged at st: instantiated once on the LDS stage and once on global scratch, so the
+    // common case addresses the stage with LDS instructions rather than flat ones
+    auto round = [&](uint32_t r, uint32_t lo, E* st) {
         const uint32_t t = r * kRound + threadIdx.x;
-        fill[threadIdx.x] = off[t] - lo;
-        __syncthreads();
-        E* st = hi - lo <= kCap ? lds_stage : (E*)S.stage + start + lo;
         for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * BLOCK) {
             uint32_t tv[kUnroll];
             E ev[kUnroll];
@@ -578,6 +576,13 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
         const uint32_t k = off[t + 1] - off[t];
         if (x < S.n && k) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2>(S, st + (off[t] - lo), k, x, T);
         __syncthreads();
+    };
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t lo = off[r * kRound], hi = off[(r + 1) * kRound];
+        fill[threadIdx.x] = off[r * kRound + threadIdx.x] - lo;
+        __syncthreads();
+        if (hi - lo <= kCap) round(r, lo, lds_stage);
+        else round(r, lo, (E*)S.stage + start + lo);
     }
     if (err) atomicOr(&S.ctl[1], err);
 }

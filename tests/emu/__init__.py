@@ -38,6 +38,7 @@ def lib():
         L.emu_raster.restype = ctypes.c_uint64
         L.emu_neurons.argtypes = [P] * 6
         L.emu_synapses.argtypes = [P] * 5
+        L.emu_sort_by_g.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32]
         _lib = L
     return _lib
 
@@ -100,3 +101,13 @@ class Emu:
                  c_match=np.zeros(e, np.uint64), c_learned=np.zeros(e, np.uint64))
         lib().emu_synapses(self.h, *[_p(o[k]) for k in ("weight", "pre_count", "c_match", "c_learned")])
         return o
+
+
+def sort_by_g(gk, w, narrow_g):
+    """The device's per-target sort of entries (gk, w), run on the CPU; returns the sorted pair.
+    narrow_g: every g is below 2^27 (the key-and-place networks may be used)."""
+    a = np.empty((len(gk), 2), np.uint32)
+    a[:, 0] = gk
+    a[:, 1] = np.asarray(w, np.float32).view(np.uint32)
+    lib().emu_sort_by_g(_p(a), len(gk), int(narrow_g))
+    return a[:, 0].copy(), a[:, 1].view(np.float32).copy()

@@ -430,4 +430,15 @@ void emu_synapses(void* p, float* w, uint64_t* pre, uint64_t* cmatch, uint64_t* 
     }
 }
 
+// the per-target sort alone, on k 8-byte entries (gk, weight bits) in place
+void emu_sort_by_g(uint32_t* gkw, uint32_t k, uint32_t narrow_g) {
+    std::vector<Entry8> ev(k);
+    for (uint32_t i = 0; i < k; ++i) ev[i] = Entry8{gkw[2 * i], u2f(gkw[2 * i + 1])};
+    sort_by_g(ev.data(), k, narrow_g != 0);
+    for (uint32_t i = 0; i < k; ++i) {
+        gkw[2 * i] = ev[i].gk;
+        gkw[2 * i + 1] = f2u(ev[i].w);
+    }
+}
+
 }  // extern "C"
