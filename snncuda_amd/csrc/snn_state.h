@@ -272,6 +272,66 @@ SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, RefsPtr refs, ui
 // which lands in the candidate's last run. That run's count is kept in a register (written back
 // once), and once a match repeats with the candidate at least as long as every learned pattern,
 // every remaining observation of the tick is a repeated match and they are applied together.
+//
+// A tick that restarts the window appends offset 0 `times` times. Every compared candidate prefix is
+// then all zeros, so every cosine is 0 (TemporalPattern.cpp:51-52) and, with a similarity threshold
+// above 0, no observation matches: the first (n = 1) only clears last_match and each later one
+// learns a copy of the candidate, a single run of n zeros. temporal_restart_tick applies those
+// learns to the ring in registers and writes each touched slot once.
+template <typename CandPtr, typename RefsPtr>
+SNN_HD uint32_t temporal_restart_tick(TemporalHdr& h, CandPtr cand, RefsPtr refs, uint32_t times,
+                                      const NeuronParams& p) {
+    cand->val[0] = 0;
+    cand->cnt[0] = times;
+    h.nruns = 1;
+    h.n = times;
+    h.head_last &= ~(1u << 8);
+    uint32_t err = 0;
+    uint32_t head = h.head_last & 0xFFu, nl = h.nlearned;
+    uint32_t len[kNeuronRefs] = {0, 0, 0, 0};
+    uint32_t written = 0;
+    bool learned = false;
+    for (uint32_t n = 2; n <= times; ++n) {
+        uint32_t hd = head;
+        if (nl >= p.max_refs) {
+            hd = (hd + 1) % kNeuronRefs;
+            nl -= 1;
+        }
+        if (nl >= kNeuronRefs) {
+            err |= kErrTemporal;
+            continue;
+        }
+        const uint32_t slot = (hd + nl) % kNeuronRefs;
+#pragma unroll
+        for (uint32_t q = 0; q < kNeuronRefs; ++q) len[q] = q == slot ? n : len[q];
+        written |= 1u << slot;
+        nl += 1;
+        head = hd;
+        learned = true;
+    }
+    h.nlearned = nl;
+    if (!learned) return err;
+    h.head_last = (h.head_last & ~0xFFu) | head;
+    uint32_t ml = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kNeuronRefs; ++q) {
+        if (written & (1u << q)) {
+            refs->ref[q].val[0] = 0;
+            refs->ref[q].cnt[0] = len[q];
+            refs->len[q] = len[q];
+            refs->nruns[q] = 1;
+        }
+        // q's place in the ring after head
+        const uint32_t pos = (q + kNeuronRefs - head) % kNeuronRefs;
+        if (pos < nl) {
+            const uint32_t l = (written & (1u << q)) ? len[q] : refs->len[q];
+            ml = l > ml ? l : ml;
+        }
+    }
+    h.maxlen = ml;
+    return err;
+}
+
 template <typename CandPtr, typename RefsPtr>
 SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, RefsPtr refs, uint32_t tick, uint32_t times,
                                       const NeuronParams& p) {
@@ -283,6 +343,7 @@ SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, RefsPtr refs
         h.n = 0;
         h.nruns = 0;
         h.trunc = 0;
+        if (!(0.0f >= p.sim_thr)) return temporal_restart_tick(h, cand, refs, times, p);
     }
     const uint32_t v = tick - h.start;
     // the run this tick's offsets go to (kRuns: none, the candidate is truncated)

@@ -39,6 +39,8 @@ def lib():
         L.emu_neurons.argtypes = [P] * 6
         L.emu_synapses.argtypes = [P] * 5
         L.emu_sort_by_g.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32]
+        L.emu_temporal_check.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32]
+        L.emu_temporal_check.restype = ctypes.c_uint32
         _lib = L
     return _lib
 
@@ -111,3 +113,11 @@ def sort_by_g(gk, w, narrow_g):
     a[:, 1] = np.asarray(w, np.float32).view(np.uint32)
     lib().emu_sort_by_g(_p(a), len(gk), int(narrow_g))
     return a[:, 0].copy(), a[:, 1].view(np.float32).copy()
+
+
+def temporal_check(ticks, times, window, sim, max_refs):
+    """Index of the first tick where the per-tick temporal pattern update differs from the same
+    observations applied one at a time (len(ticks) when they agree)."""
+    ticks = np.ascontiguousarray(ticks, np.uint32)
+    times = np.ascontiguousarray(times, np.uint32)
+    return lib().emu_temporal_check(_p(ticks), _p(times), len(ticks), window, sim, max_refs)

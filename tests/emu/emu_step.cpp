@@ -441,4 +441,32 @@ void emu_sort_by_g(uint32_t* gkw, uint32_t k, uint32_t narrow_g) {
     }
 }
 
+// One neuron's temporal pattern driven by m ticks with times[i] observations at ticks[i], once
+// through temporal_observe_tick and once through times[i] single observations (temporal_observe).
+// Returns the index of the first tick after which the two differ (header, error flags, stored
+// candidate runs, or any ring slot), or m when they agree throughout.
+uint32_t emu_temporal_check(const uint32_t* ticks, const uint32_t* times, uint32_t m, uint32_t window, float sim,
+                            uint32_t max_refs) {
+    NeuronParams p{1.0f, window, sim, max_refs};
+    TemporalHdr ha{}, hb{};
+    Runs ca{}, cb{};
+    TemporalRefs ra{}, rb{};
+    for (uint32_t i = 0; i < m; ++i) {
+        const uint32_t ea = temporal_observe_tick(ha, &ca, &ra, ticks[i], times[i], p);
+        uint32_t eb = 0;
+        Runs* pc = &cb;
+        bool dirty = false;
+        for (uint32_t t = 0; t < times[i]; ++t) eb |= temporal_observe(hb, pc, &rb, ticks[i], p, dirty);
+        bool same = ea == eb && std::memcmp(&ha, &hb, sizeof ha) == 0;
+        for (uint32_t r = 0; same && r < ha.nruns; ++r) same = ca.val[r] == cb.val[r] && ca.cnt[r] == cb.cnt[r];
+        for (uint32_t q = 0; same && q < kNeuronRefs; ++q) {
+            same = ra.len[q] == rb.len[q] && ra.nruns[q] == rb.nruns[q];
+            for (uint32_t r = 0; same && r < ra.nruns[q]; ++r)
+                same = ra.ref[q].val[r] == rb.ref[q].val[r] && ra.ref[q].cnt[r] == rb.ref[q].cnt[r];
+        }
+        if (!same) return i;
+    }
+    return m;
+}
+
 }  // extern "C"

@@ -49,3 +49,22 @@ def test_per_target_sort_orders_by_g_at_every_size(narrow_g):
             order = np.argsort(g, kind="stable")
             assert np.array_equal(sg, gk[order]), k
             assert np.array_equal(sw.view(np.uint32), w[order].view(np.uint32)), k
+
+
+@pytest.mark.parametrize("sim", [0.9, 1.0, 0.5, 0.0, -0.5])
+@pytest.mark.parametrize("max_refs", [1, 2, 4, 5])
+def test_per_tick_temporal_update_equals_single_observations(sim, max_refs):
+    # The fold applies one neuron's k observations of a tick together (a window restart in closed
+    # form, repeated matches in bulk); it must leave the header, candidate, learned ring and error
+    # flags exactly as k calls of the one-observation update. Sparse and dense ticks, long gaps
+    # (window restarts), thresholds at and below 0 (every cosine of a restart matches) and ring
+    # capacities below, at and above the four slots (the error path).
+    from tests.emu import temporal_check
+    rng = np.random.default_rng(max_refs * 10 + int(sim * 10))
+    for trial in range(40):
+        window = int(rng.choice([1, 3, 8, 32]))
+        m = 300
+        gaps = rng.choice([1, 1, 1, 2, 5, window + 1, window + 7], size=m)
+        ticks = np.cumsum(gaps).astype(np.uint32)
+        times = rng.choice([1, 1, 2, 3, 7, 12, 30], size=m).astype(np.uint32)
+        assert temporal_check(ticks, times, window, sim, max_refs) == m, (trial, window)
