@@ -376,6 +376,19 @@ __device__ void for_block_rounds(const Step& S, SegCache& c, F&& f) {
     }
 }
 
+// Entries move between registers, LDS buffers and buckets as 32-bit words in plain arrays, which stay in
+// registers (an array of 12-byte entries is kept in scratch memory instead).
+template <typename E>
+__device__ __forceinline__ void load_words(const E* p, uint32_t* w) {
+#pragma unroll
+    for (uint32_t j = 0; j < sizeof(E) / 4; ++j) __builtin_memcpy(&w[j], (const char*)p + 4 * j, 4);
+}
+template <typename E>
+__device__ __forceinline__ void store_words(E* p, const uint32_t* w) {
+#pragma unroll
+    for (uint32_t j = 0; j < sizeof(E) / 4; ++j) __builtin_memcpy((char*)p + 4 * j, &w[j], 4);
+}
+
 // Place each event in its partition's bucket: the workgroup's run in partition p starts at the
 // scanned base of (p, workgroup). Records (the event and its 16-bit target index within the
 // partition, in two arrays) are gathered per partition in LDS, `chunk` at a time, and written out
@@ -410,10 +423,14 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
     for_block_rounds<CODE_HERE, kScatterEpt>(S, cache, [&](const bool* ok, const uint64_t* v0, const uint32_t* g0,
                                                           const uint32_t* mult, const uint32_t* i) {
         uint32_t x[kScatterEpt];
-        E e[kScatterEpt];
+        uint32_t ew[kScatterEpt][sizeof(E) / 4];  // words, not entries: the array stays in registers
 #pragma unroll
-        for (uint32_t u = 0; u < kScatterEpt; ++u)
-            if (ok[u]) err |= make_event<WIDE, CODE_HERE, CODE2>(S, v0[u], g0[u], mult[u], i[u], T, x[u], e[u]);
+        for (uint32_t u = 0; u < kScatterEpt; ++u) {
+            if (!ok[u]) continue;
+            E e;
+            err |= make_event<WIDE, CODE_HERE, CODE2>(S, v0[u], g0[u], mult[u], i[u], T, x[u], e);
+            load_words(&e, ew[u]);
+        }
 #pragma unroll
         for (uint32_t u = 0; u < kScatterEpt; ++u) {
             if (!ok[u]) continue;
@@ -421,10 +438,10 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
             const uint32_t p = x[u] >> shift;
             const uint32_t slot = atomicAdd(&cnt[p], 1u);
             if (slot < ch) {
-                buf[p * ch + slot] = e[u];
+                store_words(buf + p * ch + slot, ew[u]);
                 buft[p * ch + slot] = tl;
             } else {
-                out[base[p] + slot] = e[u];
+                store_words(out + base[p] + slot, ew[u]);
                 outt[base[p] + slot] = tl;
             }
         }
@@ -522,16 +539,16 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
         __syncthreads();
         for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * BLOCK) {
             uint32_t tv[kUnroll];
-            E ev[kUnroll];
+            uint32_t ev[kUnroll][sizeof(E) / 4];
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u) {
                 const uint32_t i = i0 + u * BLOCK;
                 tv[u] = i < c ? tls[i] : kNoTarget;
-                if (i < c) ev[u] = src[i];
+                if (i < c) load_words(src + i, ev[u]);
             }
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u)
-                if (tv[u] != kNoTarget) lds_stage[atomicAdd(&aux[off[tv[u]] >> 16], 1u << 16) >> 16] = ev[u];
+                if (tv[u] != kNoTarget) store_words(lds_stage + (atomicAdd(&aux[off[tv[u]] >> 16], 1u << 16) >> 16), ev[u]);
         }
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < ntouched; j += blockDim.x) {
@@ -561,15 +578,15 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
         const uint32_t t = r * kRound + threadIdx.x;
         for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * BLOCK) {
             uint32_t tv[kUnroll];
-            E ev[kUnroll];
+            uint32_t ev[kUnroll][sizeof(E) / 4];
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u) tv[u] = i0 + u * BLOCK < c ? tls[i0 + u * BLOCK] : kNoTarget;
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u)
-                if (tv[u] / kRound == r) ev[u] = src[i0 + u * BLOCK];
+                if (tv[u] / kRound == r) load_words(src + i0 + u * BLOCK, ev[u]);
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u)
-                if (tv[u] / kRound == r) st[atomicAdd(&fill[tv[u] % kRound], 1u)] = ev[u];
+                if (tv[u] / kRound == r) store_words(st + atomicAdd(&fill[tv[u] % kRound], 1u), ev[u]);
         }
         __syncthreads();
         const uint32_t x = (p << S.pshift) + t;
