@@ -367,13 +367,22 @@ SNN_HD void key_sort_by_g(E* ev, uint32_t k) {
         key[net.lo[c]] = x < y ? x : y;
         key[net.hi[c]] = x < y ? y : x;
     }
-    E a[M];
+    // gathered as 32-bit words: an array of 12- or 16-byte entries would not stay in registers
+    constexpr uint32_t W = sizeof(E) / 4;
+    uint32_t a[M][W];
 #pragma unroll
-    for (uint32_t j = 0; j < M; ++j)
-        if (j < k) a[j] = ev[key[j] & 31u];
+    for (uint32_t j = 0; j < M; ++j) {
+        if (j >= k) continue;
+        const char* src = (const char*)(ev + (key[j] & 31u));
 #pragma unroll
-    for (uint32_t j = 0; j < M; ++j)
-        if (j < k) ev[j] = a[j];
+        for (uint32_t q = 0; q < W; ++q) __builtin_memcpy(&a[j][q], src + 4 * q, 4);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < M; ++j) {
+        if (j >= k) continue;
+#pragma unroll
+        for (uint32_t q = 0; q < W; ++q) __builtin_memcpy((char*)(ev + j) + 4 * q, &a[j][q], 4);
+    }
 }
 
 // narrow_g: every g of the tick is below 2^27
@@ -389,6 +398,13 @@ SNN_HD void sort_by_g(E* ev, uint32_t k, bool narrow_g) {
                 if (k <= 8) net_sort_by_g<8>(ev, k);
                 else net_sort_by_g<24>(ev, k);
             }
+            return;
+        }
+    } else {
+        // wider entries: smaller networks (registers), key and place only
+        if (narrow_g && k <= 16) {
+            if (k <= 8) key_sort_by_g<8>(ev, k);
+            else key_sort_by_g<16>(ev, k);
             return;
         }
     }

@@ -38,7 +38,7 @@ def lib():
         L.emu_raster.restype = ctypes.c_uint64
         L.emu_neurons.argtypes = [P] * 6
         L.emu_synapses.argtypes = [P] * 5
-        L.emu_sort_by_g.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32]
+        L.emu_sort_by_g.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         L.emu_temporal_check.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32]
         L.emu_temporal_check.restype = ctypes.c_uint32
         _lib = L
@@ -105,14 +105,13 @@ class Emu:
         return o
 
 
-def sort_by_g(gk, w, narrow_g):
-    """The device's per-target sort of entries (gk, w), run on the CPU; returns the sorted pair.
+def sort_by_g(words, narrow_g):
+    """The device's per-target sort, run on the CPU, of entries given as rows of 2, 3 or 4 uint32
+    words (8-, 12- or 16-byte entries, gk first); returns the sorted rows.
     narrow_g: every g is below 2^27 (the key-and-place networks may be used)."""
-    a = np.empty((len(gk), 2), np.uint32)
-    a[:, 0] = gk
-    a[:, 1] = np.asarray(w, np.float32).view(np.uint32)
-    lib().emu_sort_by_g(_p(a), len(gk), int(narrow_g))
-    return a[:, 0].copy(), a[:, 1].view(np.float32).copy()
+    a = np.ascontiguousarray(words, np.uint32).copy()
+    lib().emu_sort_by_g(_p(a), a.shape[0], int(narrow_g), a.shape[1])
+    return a
 
 
 def temporal_check(ticks, times, window, sim, max_refs):

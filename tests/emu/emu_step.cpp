@@ -214,6 +214,15 @@ std::vector<float> decay_table() {
 
 }  // namespace
 
+// the per-target sort on E entries held as 32-bit words
+template <typename E>
+void sort_words(uint32_t* words, uint32_t k, bool narrow_g) {
+    std::vector<E> ev(k);
+    std::memcpy(ev.data(), words, k * sizeof(E));
+    sort_by_g(ev.data(), k, narrow_g);
+    std::memcpy(words, ev.data(), k * sizeof(E));
+}
+
 extern "C" {
 
 void* emu_create(uint64_t n, const float* thr, const uint64_t* win, const float* sim, const uint64_t* mref, uint64_t e,
@@ -430,15 +439,11 @@ void emu_synapses(void* p, float* w, uint64_t* pre, uint64_t* cmatch, uint64_t* 
     }
 }
 
-// the per-target sort alone, on k 8-byte entries (gk, weight bits) in place
-void emu_sort_by_g(uint32_t* gkw, uint32_t k, uint32_t narrow_g) {
-    std::vector<Entry8> ev(k);
-    for (uint32_t i = 0; i < k; ++i) ev[i] = Entry8{gkw[2 * i], u2f(gkw[2 * i + 1])};
-    sort_by_g(ev.data(), k, narrow_g != 0);
-    for (uint32_t i = 0; i < k; ++i) {
-        gkw[2 * i] = ev[i].gk;
-        gkw[2 * i + 1] = f2u(ev[i].w);
-    }
+// the per-target sort alone, on k entries of `width` 32-bit words (gk first) in place
+void emu_sort_by_g(uint32_t* words, uint32_t k, uint32_t narrow_g, uint32_t width) {
+    if (width == 2) sort_words<Entry8>(words, k, narrow_g != 0);
+    else if (width == 3) sort_words<Entry12>(words, k, narrow_g != 0);
+    else sort_words<Entry16>(words, k, narrow_g != 0);
 }
 
 // One neuron's temporal pattern driven by m ticks with times[i] observations at ticks[i], once

@@ -30,25 +30,25 @@ def test_capacity_overflow_is_an_error_not_a_drop():
     assert m.flags & 1  # events per tick
 
 
+@pytest.mark.parametrize("width", [2, 3, 4])
 @pytest.mark.parametrize("narrow_g", [False, True])
-def test_per_target_sort_orders_by_g_at_every_size(narrow_g):
-    # sizes on both sides of the sorting networks (8, 24) and the shell sort past them; g is unique
-    # within a tick, the low 4 bits carry compartment and receptor, and each weight must travel with
-    # its event. The largest g a tick can hold is included (2^27 - 1 when the tick is narrow).
+def test_per_target_sort_orders_by_g_at_every_size(narrow_g, width):
+    # 8-, 12- and 16-byte entries at sizes on both sides of the sorting networks (8, 16, 24) and the
+    # shell sort past them; g is unique within a tick, the low 4 bits carry compartment and
+    # receptor, and every other word must travel with its event. The largest g a tick can hold is
+    # included (2^27 - 1 when the tick is narrow).
     from tests.emu import sort_by_g
-    rng = np.random.default_rng(5)
+    rng = np.random.default_rng(5 + width)
     gmax = (1 << 27) if narrow_g else (1 << 28) - 1
     for k in list(range(0, 41)) + [100, 1000]:
         for _ in range(20 if k <= 40 else 2):
             g = rng.choice(gmax, size=k, replace=False).astype(np.uint32)
             if k:
                 g[0] = gmax - 1
-            gk = (g << 4) | rng.integers(0, 16, size=k, dtype=np.uint32)
-            w = rng.standard_normal(k).astype(np.float32)
-            sg, sw = sort_by_g(gk, w, narrow_g)
-            order = np.argsort(g, kind="stable")
-            assert np.array_equal(sg, gk[order]), k
-            assert np.array_equal(sw.view(np.uint32), w[order].view(np.uint32)), k
+            rows = rng.integers(0, 1 << 32, size=(k, width), dtype=np.uint64).astype(np.uint32)
+            rows[:, 0] = (g << 4) | rng.integers(0, 16, size=k, dtype=np.uint32)
+            out = sort_by_g(rows, narrow_g)
+            assert np.array_equal(out, rows[np.argsort(g, kind="stable")]), k
 
 
 @pytest.mark.parametrize("sim", [0.9, 1.0, 0.5, 0.0, -0.5])
