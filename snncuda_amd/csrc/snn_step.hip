@@ -423,13 +423,24 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
     for_block_rounds<CODE_HERE, kScatterEpt>(S, cache, [&](const bool* ok, const uint64_t* v0, const uint32_t* g0,
                                                           const uint32_t* mult, const uint32_t* i) {
         uint32_t x[kScatterEpt];
-        uint32_t ew[kScatterEpt][sizeof(E) / 4];  // words, not entries: the array stays in registers
+        // wide entries as words, not entries: an array of 12- or 16-byte entries would not stay in
+        // registers (8-byte entries do, and keep their faster code)
+        typedef typename std::conditional<sizeof(E) == 8, E, uint32_t[sizeof(E) / 4]>::type Held;
+        Held ew[kScatterEpt];
+        auto put = [](E* dst, const Held& h) {
+            if constexpr (sizeof(E) == 8) *dst = h;
+            else store_words(dst, h);
+        };
 #pragma unroll
         for (uint32_t u = 0; u < kScatterEpt; ++u) {
             if (!ok[u]) continue;
-            E e;
-            err |= make_event<WIDE, CODE_HERE, CODE2>(S, v0[u], g0[u], mult[u], i[u], T, x[u], e);
-            load_words(&e, ew[u]);
+            if constexpr (sizeof(E) == 8) {
+                err |= make_event<WIDE, CODE_HERE, CODE2>(S, v0[u], g0[u], mult[u], i[u], T, x[u], ew[u]);
+            } else {
+                E e;
+                err |= make_event<WIDE, CODE_HERE, CODE2>(S, v0[u], g0[u], mult[u], i[u], T, x[u], e);
+                load_words(&e, ew[u]);
+            }
         }
 #pragma unroll
         for (uint32_t u = 0; u < kScatterEpt; ++u) {
@@ -438,10 +449,10 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
             const uint32_t p = x[u] >> shift;
             const uint32_t slot = atomicAdd(&cnt[p], 1u);
             if (slot < ch) {
-                store_words(buf + p * ch + slot, ew[u]);
+                put(buf + (p * ch + slot), ew[u]);
                 buft[p * ch + slot] = tl;
             } else {
-                store_words(out + base[p] + slot, ew[u]);
+                put(out + (base[p] + slot), ew[u]);
                 outt[base[p] + slot] = tl;
             }
         }
