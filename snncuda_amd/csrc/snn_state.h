@@ -647,14 +647,14 @@ SNN_HD uint32_t stdp_apply(StdpRec& r, uint32_t pre, uint32_t post, const float*
 SNN_HD void decay_to(NeuronDyn& d, uint32_t tick, const float* decay_tab) {
     if (tick > d.last_tick) {
         const uint32_t dt = tick - d.last_tick;
+        // the four factors are loaded unconditionally (an in-bounds index) so the loads overlap
+        const uint32_t at = dt < kDecayTable ? dt : 0u;
+        float f[4];
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k) {
-            if (d.cur[k] != 0.0f) {
-                const uint32_t code = (d.taus >> (2 * k)) & 3u;
-                const float f = dt < kDecayTable ? decay_tab[code * kDecayTable + dt] : 0.0f;
-                d.cur[k] = d.cur[k] * f;
-            }
-        }
+        for (uint32_t k = 0; k < 4; ++k) f[k] = decay_tab[((d.taus >> (2 * k)) & 3u) * kDecayTable + at];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k)
+            if (d.cur[k] != 0.0f) d.cur[k] = d.cur[k] * (dt < kDecayTable ? f[k] : 0.0f);
         d.last_tick = tick;
     }
 }
