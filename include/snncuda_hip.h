@@ -61,7 +61,7 @@ typedef struct snn_net_desc {
 /* Capacities (0 = default). Exceeding one during a run returns SNN_ECAPACITY: events are never
  * dropped silently. */
 typedef struct snn_options {
-    uint64_t max_events_per_tick;   /* delivered events in one tick (<= 2^28) */
+    uint64_t max_events_per_tick;   /* delivered events in one tick (< 2^28) */
     uint64_t fire_log_capacity;     /* fires held for the delay window */
     uint64_t raster_capacity;       /* fires recorded for snn_net_read_raster over the handle's life */
     int32_t track_pre_count;        /* keep per-synapse delivered counts (pre_spike_count) */
