@@ -66,7 +66,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune", type=int, default=0,
                     help="plain form: tune the launch schedule on the warm-up ticks with trials of this many ticks")
-    ap.add_argument("--schedule", default="", help="plain form: partition_shift,scatter_workgroups,graph_ticks")
+    ap.add_argument("--schedule", default="",
+                    help="plain form: partition_shift,scatter_workgroups,graph_ticks[,fold_threads]")
     ap.add_argument("--cpu-neurons", type=int, default=20_000)
     ap.add_argument("--cpu-ticks", type=int, default=200)
     ap.add_argument("--form", choices=("plain", "batched", "adjoint"), default="plain",
@@ -314,8 +315,11 @@ def main():
                     raster_capacity=1 << 20, track_pre_count=False) as dev:
         dev.run_commands(injects)
         if args.schedule:
-            ps, wg, gt = (int(x) for x in args.schedule.split(","))
-            dev.set_schedule(partition_shift=ps, scatter_workgroups=wg, graph_ticks=gt)
+            v = [int(x) for x in args.schedule.split(",")]
+            kw = dict(partition_shift=v[0], scatter_workgroups=v[1], graph_ticks=v[2])
+            if len(v) > 3:
+                kw["fold_threads"] = v[3]
+            dev.set_schedule(**kw)
         if args.autotune:
             dev.run_until(min(99, args.warmup - 1))  # past the ignition ramp first
             dev.autotune(args.autotune)
