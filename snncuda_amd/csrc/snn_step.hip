@@ -1104,8 +1104,8 @@ T* dev_alloc(std::vector<DevBuf>& owned, size_t count, bool zero = true) {
     return (T*)p;
 }
 
-template <typename T>
-T* dev_upload(std::vector<DevBuf>& owned, const std::vector<T>& v) {
+template <typename T, typename A>
+T* dev_upload(std::vector<DevBuf>& owned, const std::vector<T, A>& v) {
     T* p = dev_alloc<T>(owned, v.size(), false);
     if (!v.empty()) check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
     return p;

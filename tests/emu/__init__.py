@@ -41,6 +41,9 @@ def lib():
         L.emu_sort_by_g.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         L.emu_temporal_check.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32]
         L.emu_temporal_check.restype = ctypes.c_uint32
+        L.emu_pack_check.argtypes = [ctypes.c_uint64] + [P] * 4 + [ctypes.c_uint64] + [P] * 10 + \
+            [ctypes.c_uint32, ctypes.c_char_p, ctypes.c_uint64]
+        L.emu_pack_check.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -120,3 +123,15 @@ def temporal_check(ticks, times, window, sim, max_refs):
     ticks = np.ascontiguousarray(ticks, np.uint32)
     times = np.ascontiguousarray(times, np.uint32)
     return lib().emu_temporal_check(_p(ticks), _p(times), len(ticks), window, sim, max_refs)
+
+
+def pack_check(net, threads):
+    """The host pack on `threads` threads against the sequential reference: (code, message); code 0
+    when every output field is equal (or both raise the same message)."""
+    keep = [np.ascontiguousarray(a) for a in (
+        net.threshold, net.window, net.sim_thr, net.max_refs, net.src, net.tgt, net.weight,
+        net.max_weight, net.delay, net.comp, net.receptor, net.plastic, net.code_ptr, net.codes)]
+    msg = ctypes.create_string_buffer(512)
+    rc = lib().emu_pack_check(net.n_neurons, *[_p(a) for a in keep[:4]], net.n_synapses,
+                              *[_p(a) for a in keep[4:]], threads, msg, 512)
+    return rc, msg.value.decode()

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <random>
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "../../snncuda_amd/csrc/snn_net.h"
+#include "ref_pack.h"
 #include "../../snncuda_amd/csrc/snn_step.h"
 
 using namespace snn;
@@ -472,6 +474,36 @@ uint32_t emu_temporal_check(const uint32_t* ticks, const uint32_t* times, uint32
         if (!same) return i;
     }
     return m;
+}
+
+// The host pack on `threads` threads against the sequential reference on one network: 0 when every
+// output field is equal, or both raise the same message (written to msg); else the 1-based number of
+// the first differing field (-1: only one of them raised, -2: different messages).
+int emu_pack_check(uint64_t n, const float* thr, const uint64_t* win, const float* sim, const uint64_t* mref,
+                   uint64_t e, const uint32_t* src, const uint32_t* tgt, const float* w, const float* mw,
+                   const uint32_t* dl, const uint8_t* comp, const uint8_t* rec, const uint8_t* pl,
+                   const uint64_t* cptr, const uint32_t* codes, uint32_t threads, char* msg, uint64_t msgcap) {
+    NetDesc d{n, thr, win, sim, mref, e, src, tgt, w, mw, dl, comp, rec, pl, cptr, codes};
+    HostNet a, b;
+    std::string ea, eb;
+    try { a = build_host_net(d, threads); } catch (const std::exception& x) { ea = x.what(); }
+    try { b = build_host_net_serial(d); } catch (const std::exception& x) { eb = x.what(); }
+    std::snprintf(msg, msgcap, "%s", ea.c_str());
+    if (!ea.empty() || !eb.empty()) return ea.empty() != eb.empty() ? -1 : (ea == eb ? 0 : -2);
+    auto same = [](const auto& x, const auto& y) {
+        return x.size() == y.size() && (x.empty() || std::memcmp(x.data(), y.data(), x.size() * sizeof(x[0])) == 0);
+    };
+    const bool f[] = {a.n == b.n && a.e == b.e && a.nve == b.nve && a.lmax == b.lmax,
+                      a.ve_identity == b.ve_identity && a.any_plastic == b.any_plastic,
+                      a.expected_all == b.expected_all && a.code_compact == b.code_compact,
+                      same(a.row, b.row), same(a.grp, b.grp), same(a.ve, b.ve), same(a.tgt, b.tgt), same(a.w, b.w),
+                      same(a.meta, b.meta), same(a.delay, b.delay), same(a.expected, b.expected),
+                      same(a.slot_of, b.slot_of), same(a.tpos, b.tpos), same(a.in_ptr, b.in_ptr),
+                      same(a.stdp, b.stdp), same(a.tpos_conn, b.tpos_conn), same(a.wmax, b.wmax),
+                      same(a.params, b.params)};
+    for (size_t i = 0; i < sizeof f; ++i)
+        if (!f[i]) return (int)i + 1;
+    return 0;
 }
 
 }  // extern "C"
