@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 #include <utility>
@@ -134,8 +135,11 @@ struct PackedConnectome {
 // Connectome: .neurons[i].{id, params.{threshold, pattern_window_ticks, similarity_threshold,
 // max_reference_patterns}}, .synapses[j].{source, target, weight, max_weight, delay_ticks,
 // spike_code_offsets, compartment, receptor, plasticity_enabled}.
+// The synapse passes run on `threads` threads (0: the machine's, at most 16; one below 65,536
+// synapses), each over a contiguous range of synapses; the arrays and the error raised (that of the
+// first offending synapse) are the same at any thread count.
 template <class Connectome>
-PackedConnectome pack(const Connectome& c) {
+PackedConnectome pack(const Connectome& c, unsigned threads = 0) {
     PackedConnectome p;
     const std::size_t n = c.neurons.size();
     const std::size_t e = c.synapses.size();
@@ -165,26 +169,57 @@ PackedConnectome pack(const Connectome& c) {
     p.plastic.resize(e);
     p.code_ptr.resize(e + 1);
     p.code_ptr[0] = 0;
-    for (std::size_t j = 0; j < e; ++j) {
-        const auto& s = c.synapses[j];
-        const auto src = p.index.find(static_cast<std::uint64_t>(s.source));
-        const auto tgt = p.index.find(static_cast<std::uint64_t>(s.target));
-        if (src == p.index.end() || tgt == p.index.end()) {
-            throw Error(SNN_EINVAL, "pack: synapse " + std::to_string(j) + " names a neuron id not in the connectome");
-        }
-        p.source[j] = src->second;
-        p.target[j] = tgt->second;
-        p.weight[j] = s.weight;
-        p.max_weight[j] = s.max_weight;
-        p.delay[j] = s.delay_ticks;
-        p.compartment[j] = static_cast<std::uint8_t>(s.compartment);
-        p.receptor[j] = static_cast<std::uint8_t>(s.receptor);
-        p.plastic[j] = s.plasticity_enabled ? 1 : 0;
-        for (const auto off : s.spike_code_offsets) {
-            p.codes.push_back(static_cast<std::uint32_t>(off));
-        }
-        p.code_ptr[j + 1] = p.codes.size();
+    unsigned T = threads;
+    if (T == 0) {
+        const unsigned hw = std::thread::hardware_concurrency();
+        T = e < (1u << 16) ? 1u : std::max(1u, std::min(hw, 16u));
     }
+    auto run = [T](auto&& fn) {
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < T; ++t) th.emplace_back([&fn, t] { fn(t); });
+        fn(0u);
+        for (auto& x : th) x.join();
+    };
+    // per synapse: dense endpoints and fields, and the code count in code_ptr[j + 1]; the index map
+    // is only read here
+    std::vector<std::size_t> bad(T, e);
+    std::vector<std::uint64_t> ncodes(T + 1, 0);
+    run([&](unsigned t) {
+        std::uint64_t cnt = 0;
+        for (std::size_t j = e * t / T; j < e * (t + 1) / T; ++j) {
+            const auto& s = c.synapses[j];
+            const auto src = p.index.find(static_cast<std::uint64_t>(s.source));
+            const auto tgt = p.index.find(static_cast<std::uint64_t>(s.target));
+            if (src == p.index.end() || tgt == p.index.end()) {
+                bad[t] = j;
+                break;
+            }
+            p.source[j] = src->second;
+            p.target[j] = tgt->second;
+            p.weight[j] = s.weight;
+            p.max_weight[j] = s.max_weight;
+            p.delay[j] = s.delay_ticks;
+            p.compartment[j] = static_cast<std::uint8_t>(s.compartment);
+            p.receptor[j] = static_cast<std::uint8_t>(s.receptor);
+            p.plastic[j] = s.plasticity_enabled ? 1 : 0;
+            p.code_ptr[j + 1] = s.spike_code_offsets.size();
+            cnt += s.spike_code_offsets.size();
+        }
+        ncodes[t + 1] = cnt;
+    });
+    const std::size_t first_bad = *std::min_element(bad.begin(), bad.end());
+    if (first_bad < e) {
+        throw Error(SNN_EINVAL, "pack: synapse " + std::to_string(first_bad) + " names a neuron id not in the connectome");
+    }
+    for (unsigned t = 0; t < T; ++t) ncodes[t + 1] += ncodes[t];
+    p.codes.resize(ncodes[T]);
+    run([&](unsigned t) {
+        std::uint64_t at = ncodes[t];
+        for (std::size_t j = e * t / T; j < e * (t + 1) / T; ++j) {
+            for (const auto off : c.synapses[j].spike_code_offsets) p.codes[at++] = static_cast<std::uint32_t>(off);
+            p.code_ptr[j + 1] = at;
+        }
+    });
     return p;
 }
 

@@ -65,29 +65,39 @@ bool same(const std::vector<T>& a, const std::vector<T>& b) {
 
 int cmd_pack(const netfile::Net& t) {
     const auto c = netfile::connectome(t);
-    const auto p = snncuda_hip::pack(c);
-    bool ok = same(p.threshold, t.thr) && same(p.window, t.win) && same(p.sim_thr, t.sim) &&
-              same(p.max_refs, t.mref) && same(p.source, t.src) && same(p.target, t.tgt) && same(p.weight, t.w) &&
-              same(p.max_weight, t.mw) && same(p.delay, t.dl) && same(p.compartment, t.comp) &&
-              same(p.receptor, t.rec) && same(p.plastic, t.pl) && same(p.code_ptr, t.cptr) &&
-              same(p.codes, t.codes);
-    const auto d = p.desc();
-    ok = ok && d.n_neurons == t.n && d.n_synapses == t.e;
-    if (!ok) {
-        std::fprintf(stderr, "pack differs from the file's arrays\n");
-        return 1;
+    // the automatic thread count and forced ones (several threads over a few synapses each)
+    for (const unsigned threads : {0u, 1u, 3u, 16u}) {
+        const auto p = snncuda_hip::pack(c, threads);
+        bool ok = same(p.threshold, t.thr) && same(p.window, t.win) && same(p.sim_thr, t.sim) &&
+                  same(p.max_refs, t.mref) && same(p.source, t.src) && same(p.target, t.tgt) &&
+                  same(p.weight, t.w) && same(p.max_weight, t.mw) && same(p.delay, t.dl) &&
+                  same(p.compartment, t.comp) && same(p.receptor, t.rec) && same(p.plastic, t.pl) &&
+                  same(p.code_ptr, t.cptr) && same(p.codes, t.codes);
+        const auto d = p.desc();
+        ok = ok && d.n_neurons == t.n && d.n_synapses == t.e;
+        if (!ok) {
+            std::fprintf(stderr, "pack on %u threads differs from the file's arrays\n", threads);
+            return 1;
+        }
     }
     if (t.e > 0) {
+        // the first unknown id in connectome order is the one reported, at any thread count
         auto bad = c;
-        bad.synapses[0].target = 0;  // invalid_id: no neuron carries it
-        try {
-            (void)snncuda_hip::pack(bad);
-            std::fprintf(stderr, "pack accepted a synapse with an unknown target\n");
-            return 1;
-        } catch (const snncuda_hip::Error& err) {
-            if (err.code() != SNN_EINVAL) {
-                std::fprintf(stderr, "pack raised code %d, expected SNN_EINVAL\n", err.code());
+        const std::size_t first = t.e / 2;
+        bad.synapses[first].target = 0;  // invalid_id: no neuron carries it
+        bad.synapses[t.e - 1].source = 0;
+        const std::string want = "synapse " + std::to_string(first) + " names";
+        for (const unsigned threads : {1u, 3u, 16u}) {
+            try {
+                (void)snncuda_hip::pack(bad, threads);
+                std::fprintf(stderr, "pack accepted a synapse with an unknown target\n");
                 return 1;
+            } catch (const snncuda_hip::Error& err) {
+                if (err.code() != SNN_EINVAL || std::string(err.what()).find(want) == std::string::npos) {
+                    std::fprintf(stderr, "pack raised code %d (%s), expected SNN_EINVAL for %s\n", err.code(),
+                                 err.what(), want.c_str());
+                    return 1;
+                }
             }
         }
     }
