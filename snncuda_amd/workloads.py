@@ -329,6 +329,17 @@ def config1(seed: int = 0, n: int = 100_000, fanout: int = 1000, n_sources: int 
     return net
 
 
+def config1_spec(seed: int = 0, n: int = 100_000, fanout: int = 1000, n_sources: int = 1000, source_fanout: int = 1000,
+                 ticks: int = 10_000, threshold: float = 1.1, plastic=True):
+    """The config-1 network as a device-generator spec plus its pattern-replay commands, without
+    building the synapse arrays on the host (equal to config1(...) built there)."""
+    spec = dict(n=n, n_sources=n_sources, fanout=np.full(n, fanout, np.int64), source_fanout=source_fanout, seed=seed,
+                n_exc=int(round(n * 0.8)), w_exc=0.5, w_inh=1.0, delay_max=20,
+                compartments=(SOMA, BASAL, APICAL, INHIBITORY), plastic=PLASTIC_MODES[plastic], threshold=threshold,
+                window=32, sim_thr=0.9, max_refs=1, n_codes=4, code_max=7)
+    return spec, pattern_commands(seed, n, n_sources, ticks)
+
+
 def config1_reduced(seed: int = 0, ticks: int = 300, threshold: float = 4.5, plastic: bool = True) -> Net:
     """Config 1's law at 2,000 neurons (fan-out 100, 100 input neurons)."""
     return config1(seed, n=2000, fanout=100, n_sources=100, source_fanout=100, ticks=ticks, threshold=threshold,

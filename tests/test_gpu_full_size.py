@@ -55,3 +55,48 @@ def test_full_size_results_do_not_depend_on_the_schedule(stdp):
         assert np.array_equal(a["raster"], b["raster"]), sc
         for k in ("membrane", "current", "spikes"):
             assert np.array_equal(a[k], b[k]), (sc, k)
+
+
+TICKS1 = 95
+
+
+def _run_digest(spec, injects, schedule):
+    import hashlib
+    with sa.Network(spec=spec, max_events_per_tick=1 << 27, fire_log_capacity=1 << 24, raster_capacity=1 << 24,
+                    track_pre_count=False) as dev:
+        dev.run_commands(injects)
+        if schedule is not None:
+            dev.set_schedule(partition_shift=schedule[0], scatter_workgroups=schedule[1], graph_ticks=schedule[2],
+                             fold_threads=schedule[3])
+        dev.run_until(TICKS1 - 1)
+        nrn = dev.neurons()
+        syn = dev.synapses()
+        d, f = dev.tick_counts(0, TICKS1)
+        out = dict(raster=dev.raster(), delivered=dev.delivered, fired=dev.fired, per_tick=(d, f))
+        out.update({"n_" + k: hashlib.sha1(np.ascontiguousarray(v).tobytes()).hexdigest() for k, v in nrn.items()})
+        out.update({"s_" + k: hashlib.sha1(np.ascontiguousarray(v).tobytes()).hexdigest() for k, v in syn.items()})
+        return out
+
+
+def test_full_size_spike_code_results_do_not_depend_on_the_schedule():
+    # The spike-code workload at full size: 100,000 neurons, fan-out 1,000 (1.01e8 synapses with the
+    # input neurons'), 4 code offsets per synapse, 4 compartments, STDP on every synapse, pattern
+    # replay over background input. At threshold 4.0 activity grows from ~2e4 to ~1e6 delivered
+    # events per tick by tick 95 (it runs away a few ticks later), so the run covers sparse and
+    # dense folds, per-event code patterns and STDP. Every neuron and synapse field (digests) must
+    # be bit-identical across partition sizes, scatter workgroups, graph replay and fold sizes.
+    spec, cmds = W.config1_spec(0, ticks=TICKS1, threshold=4.0)
+    injects = [c for c in cmds if c[0] == 0]
+    a = _run_digest(spec, injects, None)
+    d, f = a["per_tick"]
+    print(f"  spike-code net: {a['delivered']} delivered, {a['fired']} fired in {TICKS1} ticks; "
+          f"last ticks {d[-5:].tolist()}", flush=True)
+    assert d[-5:].mean() > 1e5
+    assert int(d.sum()) == a["delivered"] and int(f.sum()) == a["fired"]
+    for sc in ((10, 256, 0, 1024), (11, 512, 8, 256)):
+        b = _run_digest(spec, injects, sc)
+        assert a["delivered"] == b["delivered"] and a["fired"] == b["fired"], sc
+        assert np.array_equal(a["raster"], b["raster"]), sc
+        for k in a:
+            if k.startswith(("n_", "s_")):
+                assert a[k] == b[k], (sc, k)
