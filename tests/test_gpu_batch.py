@@ -148,3 +148,47 @@ def test_device_generated_network_with_codes_equals_host_build():
         ds, rs = dev.synapses(), ref.synapses()
         for key in ("weight", "pre_count", "c_match", "c_learned", "last_pre", "last_post"):
             assert np.array_equal(ds[key], rs[key].astype(ds[key].dtype)), key
+
+
+def test_full_size_batch_equals_each_network_run_alone():
+    # The 512-network workload at full size (neurons log-uniform in [500, 50,000], fan-out U{50..500},
+    # 2 code offsets, STDP on; about 5.5e6 neurons and 1.5e9 synapses in all), generated on the
+    # device and stepped together under the online autotuner: a network's raster, fire count, every
+    # neuron field and its STDP weights (the union's synapse fields would be tens of GB on the host)
+    # must equal that network run alone, for the smallest, the largest and a spread of the others.
+    from snncuda_amd import workloads as W
+    T = 150
+    specs, cmds = zip(*[W.config3_spec(s, ticks=T) for s in range(512)])
+
+    def weights(net):
+        w = np.zeros(net.e, np.float32)
+        sa._check(sa.lib().snn_net_read_synapses(net._h, sa._p(w), None, None, None, None, None, None))
+        return w
+
+    with sa.Batch(specs=list(specs), track_pre_count=False, max_events_per_tick=1 << 24,
+                  fire_log_capacity=1 << 22, raster_capacity=1 << 22) as b:
+        b.union.autotune(8)
+        b.run_commands(list(cmds))
+        raster = b.union.raster()
+        _, fd = b.net_counts()
+        sizes = np.array([sp["n"] for sp in specs])
+        pick = sorted({int(sizes.argmin()), int(sizes.argmax()), 0, 1, 137, 300, 511})
+        print(f"batch: {int(b.neuron_off[-1])} neurons, {int(b.synapse_off[-1])} synapses; "
+              f"{int(fd.sum())} fires; schedule {b.union.schedule()}; checking networks {pick}", flush=True)
+        assert int(fd.sum()) > 0
+        nrn = b.union.neurons()
+        w = weights(b.union)
+        got = {k: (b.net_raster(k, raster), b.net_neurons(k, nrn), b.net_synapses(k, {"weight": w})["weight"],
+                   int(fd[k])) for k in pick}
+        del nrn, w
+    for k in pick:
+        with sa.Network(spec=specs[k], track_pre_count=False, max_events_per_tick=1 << 22,
+                        fire_log_capacity=1 << 20, raster_capacity=1 << 20) as net:
+            net.run_commands(cmds[k])
+            r, nn, ww, f = got[k]
+            assert net.fired == f, k
+            assert np.array_equal(net.raster(), r), k
+            alone = net.neurons()
+            for key in nn:
+                assert np.array_equal(nn[key], alone[key]), (k, key)
+            assert np.array_equal(ww, weights(net)), k
