@@ -52,7 +52,6 @@ enum : uint32_t {
     kErrCode = 1u << 4,          // a synapse code pattern exceeded learned capacity
     kErrRaster = 1u << 5,        // raster capacity exceeded
     kErrCounter = 1u << 6,       // a 28-bit match counter overflowed
-    kErrSort = 1u << 7,          // per-target event list exceeded the sort scratch
 };
 
 // Dendritic compartment and receptor enums: include/snncuda/snn/Synapse.h:9-21

@@ -2321,7 +2321,6 @@ int snn_net_run_until(snn_net* net, uint64_t inclusive_tick) {
             if (ctl[1] & kErrCode) m += " synapse-code-pattern";
             if (ctl[1] & kErrRaster) m += " raster";
             if (ctl[1] & kErrCounter) m += " match-counter";
-            if (ctl[1] & kErrSort) m += " sort";
             return fail(SNN_ECAPACITY, m);
         }
         return SNN_OK;
