@@ -1203,7 +1203,9 @@ template <bool WIDE, bool CODE_HERE, bool CODE2>
 void launch_scatter(snn_net* net) {
     typedef typename EntryOf<WIDE, CODE_HERE>::type E;
     Step& S = net->S;
-    const uint32_t want = WIDE ? 8u : 16u;  // 128 bytes of events per flush
+    // up to 256 bytes of events per flush, as many as the LDS holds (26 per partition at 244
+    // partitions): longer flushed runs leave fewer partly written cache lines
+    const uint32_t want = WIDE ? 16u : 32u;
     const uint64_t room = kScatterDynLds > 8ull * S.nparts ? kScatterDynLds - 8ull * S.nparts : 0;
     constexpr uint64_t kRecBytes = sizeof(E) + sizeof(uint16_t);
     uint32_t ch = (uint32_t)std::min<uint64_t>(want, room / ((uint64_t)S.nparts * kRecBytes));
