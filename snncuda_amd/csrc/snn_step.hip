@@ -508,6 +508,9 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
     __shared__ uint32_t aux[kAux];
     extern __shared__ uint32_t off[];  // PS + 1: event counts per target, then their places
     const uint32_t PS = 1u << S.pshift;
+    // counters per thread in the scans over the partition's targets: 1,024 per block scan (the
+    // partition holds 1,024 to 4,096 targets)
+    constexpr uint32_t kScanPer = 1024 / BLOCK;
     const uint32_t p = blockIdx.x;
     const uint32_t T = S.ctl[0];
     const uint32_t start = S.hist[(uint64_t)p * S.nwg];
@@ -536,13 +539,22 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
         // targets only, one per thread. One scan gives both the targets' places (low 16 bits) and
         // the touched targets' ranks (high 16 bits); c <= kCap < 2^16 keeps the halves apart.
         uint32_t run = 0;
-        for (uint32_t t0 = 0; t0 < PS; t0 += blockDim.x) {
-            const uint32_t t = t0 + threadIdx.x;
-            const uint32_t v = off[t];
+        for (uint32_t t0 = 0; t0 < PS; t0 += kScanPer * BLOCK) {
+            const uint32_t tb = t0 + threadIdx.x * kScanPer;
+            uint32_t v[kScanPer], sum = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < kScanPer; ++q) {
+                v[q] = off[tb + q];
+                sum += v[q] | (v[q] ? 1u << 16 : 0u);
+            }
             uint32_t tot;
-            const uint32_t ex = run + block_excl_scan_n<BLOCK>(v | (v ? 1u << 16 : 0u), &tot);
-            off[t] = ex;
-            if (v) aux[ex >> 16] = t | (ex << 16);
+            uint32_t ex = run + block_excl_scan_n<BLOCK>(sum, &tot);
+#pragma unroll
+            for (uint32_t q = 0; q < kScanPer; ++q) {
+                off[tb + q] = ex;
+                if (v[q]) aux[ex >> 16] = (tb + q) | (ex << 16);
+                ex += v[q] | (v[q] ? 1u << 16 : 0u);
+            }
             run += tot;
         }
         if (threadIdx.x == 0) off[PS] = run;
@@ -572,11 +584,21 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
         return;
     }
     uint32_t run = 0;
-    for (uint32_t t0 = 0; t0 < PS; t0 += blockDim.x) {
-        const uint32_t v = off[t0 + threadIdx.x];
+    for (uint32_t t0 = 0; t0 < PS; t0 += kScanPer * BLOCK) {
+        const uint32_t tb = t0 + threadIdx.x * kScanPer;
+        uint32_t v[kScanPer], sum = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kScanPer; ++q) {
+            v[q] = off[tb + q];
+            sum += v[q];
+        }
         uint32_t tot;
-        const uint32_t ex = block_excl_scan_n<BLOCK>(v, &tot);
-        off[t0 + threadIdx.x] = run + ex;
+        uint32_t ex = run + block_excl_scan_n<BLOCK>(sum, &tot);
+#pragma unroll
+        for (uint32_t q = 0; q < kScanPer; ++q) {
+            off[tb + q] = ex;
+            ex += v[q];
+        }
         run += tot;
     }
     if (threadIdx.x == 0) off[PS] = run;
