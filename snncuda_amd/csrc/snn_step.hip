@@ -258,12 +258,14 @@ __global__ void __launch_bounds__(kBlock) k_segments(Step S) {
 // Event ranges: workgroup w owns events [D*w/nwg, D*(w+1)/nwg) (a contiguous range of g), caches the
 // descriptors of the segments covering it in LDS and lets its lanes walk consecutive events, so the
 // CSR reads are coalesced and the load is balanced whatever the fan-out distribution.
-constexpr uint32_t kSegChunk = 1024;
+constexpr uint32_t kSegChunk = 512;
 
+constexpr uint32_t kSegTab = 1024;  // segment lookup table entries per chunk
 struct SegCache {
     uint32_t off[kSegChunk + 1];
     uint64_t v0[kSegChunk];
     uint32_t mult[kSegChunk];
+    uint32_t tab[kSegTab];  // the segment of every 2^shift-th position of the chunk's range
 };
 
 // last segment whose first event is <= g (segments of zero events share their first position)
@@ -274,6 +276,26 @@ __device__ __forceinline__ uint32_t seg_find(const uint32_t* off, uint32_t n, ui
         if (off[mid] <= g) lo = mid + 1; else hi = mid;
     }
     return lo - 1;
+}
+
+// Positions [ga, gb) of a cached chunk of nq segments (off[0] <= ga, gb <= off[nq]) take their
+// segment from a table of every 2^shift-th position's segment and a short forward walk, instead of
+// a binary search each: a block builds the table once per chunk (kSegTab searches at most).
+__device__ __forceinline__ uint32_t seg_tab_shift(uint32_t ga, uint32_t gb) {
+    const uint32_t span = ga < gb ? gb - ga : 0u;
+    uint32_t sh = 0;
+    while ((span >> sh) >= kSegTab) ++sh;  // ceil(span / 2^sh) <= kSegTab
+    return sh;
+}
+__device__ __forceinline__ void seg_tab_build(SegCache& c, uint32_t nq, uint32_t ga, uint32_t gb, uint32_t sh) {
+    const uint32_t nent = ga < gb ? ((gb - ga + (1u << sh) - 1) >> sh) : 0u;
+    for (uint32_t j = threadIdx.x; j < nent; j += blockDim.x) c.tab[j] = seg_find(c.off, nq, ga + (j << sh));
+    __syncthreads();
+}
+__device__ __forceinline__ uint32_t seg_lookup(const SegCache& c, uint32_t ga, uint32_t sh, uint32_t g) {
+    uint32_t k = c.tab[(g - ga) >> sh];
+    while (c.off[k + 1] <= g) ++k;  // stops at nq - 1 at the latest: g < off[nq]
+    return k;
 }
 
 // Per-workgroup histogram of the tick's events over target partitions (LDS atomics only).
@@ -305,6 +327,8 @@ __global__ void __launch_bounds__(kPBlock) k_pcount(Step S) {
         __syncthreads();
         const uint32_t ga = max(g0, cache.off[0]);
         const uint32_t gb = min(g1, cache.off[nq]);
+        const uint32_t sh = seg_tab_shift(ga, gb);
+        seg_tab_build(cache, nq, ga, gb, sh);
         for (uint32_t g = ga + threadIdx.x; g < gb; g += kCountUnroll * blockDim.x) {
             uint32_t x[kCountUnroll];
 #pragma unroll
@@ -312,7 +336,7 @@ __global__ void __launch_bounds__(kPBlock) k_pcount(Step S) {
                 const uint32_t gg = g + u * blockDim.x;
                 x[u] = 0xFFFFFFFFu;
                 if (gg < gb) {
-                    const uint32_t k = seg_find(cache.off, nq, gg);
+                    const uint32_t k = seg_lookup(cache, ga, sh, gg);
                     x[u] = event_target(S, cache.v0[k], gg - cache.off[k]);
                 }
             }
@@ -354,6 +378,8 @@ __device__ void for_block_rounds(const Step& S, SegCache& c, F&& f) {
         __syncthreads();
         const uint32_t ga = max(g0, c.off[0]);
         const uint32_t gb = min(g1, c.off[nq]);
+        const uint32_t sh = seg_tab_shift(ga, gb);
+        seg_tab_build(c, nq, ga, gb, sh);
         for (uint32_t gr = ga; gr < gb; gr += EPT * blockDim.x) {
             bool ok[EPT];
             uint64_t v0[EPT];
@@ -362,7 +388,7 @@ __device__ void for_block_rounds(const Step& S, SegCache& c, F&& f) {
             for (uint32_t u = 0; u < EPT; ++u) {
                 const uint32_t g = gr + u * blockDim.x + threadIdx.x;
                 ok[u] = g < gb;
-                const uint32_t k = ok[u] ? seg_find(c.off, nq, g) : 0u;
+                const uint32_t k = ok[u] ? seg_lookup(c, ga, sh, g) : 0u;
                 v0[u] = ok[u] ? c.v0[k] : 0ull;
                 o[u] = ok[u] ? c.off[k] : 0u;
                 m[u] = (ok[u] && WITH_MULT) ? c.mult[k] : 0u;
@@ -1217,7 +1243,7 @@ void launch_scan(snn_net* net, Src src, uint32_t* out, uint32_t* total, uint64_t
                        out, total, cap, err_flag, &net->S.ctl[1], zero_on_overflow);
 }
 
-// LDS of the scatter besides its 16 KB segment cache: base and count per partition plus the
+// LDS of the scatter besides its 12 KB segment cache: base and count per partition plus the
 // write-combining buffers; two scatter workgroups fit in a CU's 160 KB
 constexpr uint32_t kScatterDynLds = 64 * 1024;
 
