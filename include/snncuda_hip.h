@@ -223,9 +223,12 @@ int snn_adjoint_read_events(snn_adjoint* adj, uint64_t* ptr, uint32_t* tick, uin
 int snn_adjoint_forward(snn_adjoint* adj, const float* d_weight, const float* params, float* d_trace);
 int snn_adjoint_backward(snn_adjoint* adj, const float* d_grad_trace, float* d_grad_weight, double* grad_params);
 
-/* Mean duration (ms) of each step kernel over the last run, measured with HIP events on the
- * handle's stream when profiling is enabled (snn_net_set_profiling). names receives a
- * comma-separated list. Returns the number of kernels written. */
+/* Per-kernel time of the step, measured with HIP events on the handle's stream while profiling is
+ * enabled (no counterpart in the reference). snn_net_set_profiling(enabled): 0 off, 1 every kernel,
+ * 0x40000000 | mask only the kernels whose bit is set (bit k = the k-th name that
+ * snn_net_kernel_times reports), so a timed run carries events around those kernels only; it also
+ * clears the totals. snn_net_kernel_times: total ms and launches per kernel since then; names
+ * receives a comma-separated list. Returns the number of kernels written. */
 int snn_net_set_profiling(snn_net* net, int enabled);
 int snn_net_kernel_times(snn_net* net, double* ms_total, uint64_t* launches, int cap, char* names, uint64_t names_cap);
 

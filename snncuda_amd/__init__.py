@@ -355,8 +355,18 @@ class Network:
         lib().snn_net_tune_log(self._h, buf, 1 << 16)
         return buf.value.decode()
 
-    def set_profiling(self, on: bool) -> None:
-        _check(lib().snn_net_set_profiling(self._h, int(on)))
+    KERNELS = ("plan", "segments", "scan_segments", "count", "scan_partitions", "scatter", "fold", "scan_fires",
+               "fire_write", "post", "advance")
+
+    def set_profiling(self, on: bool, kernels=None) -> None:
+        """Time the step kernels with HIP events (all of them, or only those named in `kernels`);
+        clears the totals that kernel_times() reports."""
+        mode = int(bool(on))
+        if on and kernels is not None:
+            mode = 0x40000000
+            for k in kernels:
+                mode |= 1 << self.KERNELS.index(k)
+        _check(lib().snn_net_set_profiling(self._h, mode))
 
     def kernel_times(self) -> Dict[str, tuple]:
         """{kernel: (total ms, launches)} over runs since set_profiling(True)."""

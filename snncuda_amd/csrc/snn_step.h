@@ -54,17 +54,18 @@ SNN_HD float u2f(uint32_t u) {
 // event entries in a target's bin
 struct Entry8 {            // no STDP, code patterns updated edge-parallel
     uint32_t gk;           // g << 4 | compartment << 2 | receptor
-    float w;
+    float w;               // the receptor current (weight * gain) * sign
 };
 struct Entry12 {           // STDP, code patterns updated edge-parallel
     uint32_t gk;
-    float w;               // weight; for a plastic event (its weight is in the STDP record) the
-                           // bits of the synapse's delay
+    float w;               // the receptor current; for a plastic event (its weight is in the STDP
+                           // record) the bits of the synapse's delay
     uint32_t tpos;         // STDP record (kNone: not plastic or external)
 };
 struct Entry16 {           // code patterns updated per event in the fold
     uint32_t gk;
-    float w;               // weight
+    float w;               // the receptor current of the static weight (a plastic event's current
+                           // comes from its STDP record)
     uint32_t tpos;         // STDP record (kNone: not plastic or external)
     uint32_t slot;         // synapse slot (kNone: external)
 };
@@ -258,7 +259,7 @@ SNN_HD uint32_t make_event(const Step& S, uint64_t v0, uint32_t g0, uint32_t mul
     if (v0 & kInjFlag) {
         const uint64_t k = (v0 & ~kInjFlag) + i;
         x = S.inj_tgt[k];
-        e.w = S.inj_w[k];
+        e.w = S.inj_w[k] * receptor_gain(kAmpa) * receptor_sign(kAmpa);
         e.gk = (g << 4) | (kSoma << 2) | kAmpa;  // process_external_input: Soma, Ampa
         if constexpr (WIDE) e.tpos = kNone;
         if constexpr (WIDE && !CODE_HERE) e.slot = kNone;
@@ -268,7 +269,10 @@ SNN_HD uint32_t make_event(const Step& S, uint64_t v0, uint32_t g0, uint32_t mul
     const uint32_t slot = S.ve ? S.ve[v] : (uint32_t)v;
     x = S.tgt[slot];
     const uint32_t m = S.meta[slot];
-    e.w = S.w[slot];
+    // the receptor current (w * gain) * sign of process_input (SynapseDendriteProcessor.cpp:183),
+    // formed here once per event with the same operations; a plastic event's weight is read from its
+    // STDP record in the fold instead
+    e.w = S.w[slot] * receptor_gain(m & 3u) * receptor_sign(m & 3u);
     e.gk = (g << 4) | (m & 15u);
     if constexpr (WIDE) {
         e.tpos = (m & 16u) ? S.tpos[slot] : kNone;
@@ -385,11 +389,18 @@ SNN_HD void key_sort_by_g(E* ev, uint32_t k) {
     }
 }
 
-// narrow_g: every g of the tick is below 2^27
-template <typename E>
+// narrow_g: every g of the tick is below 2^27. SMALL: sorting networks of 8 entries at most (fewer
+// registers; larger pieces take the in-place shell sort).
+template <bool SMALL = false, typename E>
 SNN_HD void sort_by_g(E* ev, uint32_t k, bool narrow_g) {
     if (k <= 1) return;
-    if constexpr (sizeof(E) == sizeof(Entry8)) {
+    if constexpr (SMALL) {
+        if (k <= 8) {
+            if (narrow_g) key_sort_by_g<8>(ev, k);
+            else net_sort_by_g<8>(ev, k);
+            return;
+        }
+    } else if constexpr (sizeof(E) == sizeof(Entry8)) {
         if (k <= 24) {
             if (narrow_g) {
                 if (k <= 8) key_sort_by_g<8>(ev, k);
@@ -441,12 +452,164 @@ SNN_HD StdpTally post_catch_up(const Step& S, StdpRec& r, uint32_t T, uint32_t f
     return changed;
 }
 
-// The sequential fold of target x's k events at tick T (ev is the target's own array, in any
-// order; it is sorted by g here). Returns error flags.
+// One target's fold within a tick, kept in registers while its events arrive in pieces: every
+// piece holds events of larger g than the pieces before it (the fold reads a partition's bucket
+// in runs of whole scatter workgroups, and each scatter workgroup owns a contiguous range of g).
+// Only what the tick's events change is carried: the compartments decayed to T, the tau codes and
+// the fire count; the rest of the neuron state is re-read at fold_end.
+struct FoldTarget {
+    float cur[4];          // compartment currents, decayed to T at fold_begin
+    uint32_t taus_fires;   // bits 0-7 receptor code per compartment, bits 8-31 fires this tick
+    uint32_t k;            // events applied so far this tick
+    uint32_t first_g;      // g of the first fire
+    float threshold;
+};
+constexpr uint32_t kFoldMaxFires = (1u << 24) - 1;
+
+// Decaying at fold_begin instead of at the first event gives the same values (decay_to depends only
+// on T and the stored state); a target without events this tick is never written back.
+SNN_HD void fold_begin(const Step& S, FoldTarget& f, uint32_t x, uint32_t T) {
+    NeuronDyn d = S.dyn[x];
+    decay_to(d, T, S.decay_tab);
+    f.cur[0] = d.cur[0];
+    f.cur[1] = d.cur[1];
+    f.cur[2] = d.cur[2];
+    f.cur[3] = d.cur[3];
+    f.taus_fires = d.taus & 0xFFu;
+    f.k = 0;
+    f.first_g = 0;
+    f.threshold = S.np[x].threshold;
+}
+
+// The next kc events of target x at tick T (ev: this piece, any order; sorted by g here), applied
+// in order: receptor current, membrane, threshold / reset and the pre-synaptic side of STDP
+// (process_synaptic_input / process_input, SynapseDendriteProcessor.cpp:89-212). Returns error flags.
+// narrow_g: every g of the tick is below 2^27 (read once per kernel).
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2, typename E>
-SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32_t T) {
+SNN_HD uint32_t fold_piece(const Step& S, FoldTarget& f, E* ev, uint32_t kc, uint32_t x, uint32_t T, bool narrow_g) {
+    if (kc == 0) return 0;
+    sort_by_g<true>(ev, kc, narrow_g);
+    uint32_t err = 0;
+    for (uint32_t i = 0; i < kc; ++i) {
+        const E e = ev[i];
+        const uint32_t g = e.gk >> 4;
+        const uint32_t comp = (e.gk >> 2) & 3u;
+        const uint32_t rec = e.gk & 3u;
+        float cur = e.w;  // the receptor current, formed in make_event
+        if constexpr (WIDE) {
+            if (STDP && e.tpos != kNone) {
+                StdpRec r = S.stdp[e.tpos];
+                StdpTally changed = post_catch_up(S, r, T, f.taus_fires >> 8);
+                uint32_t dl;
+                if constexpr (CODE_HERE) dl = f2u(e.w);
+                else dl = S.delay[e.slot];
+                const uint32_t pre = T >= dl ? T - dl : T;
+                if (r.last_post != kNone) changed.add(stdp_apply(r, pre, r.last_post, S.ltp, S.ltd, S.wmax));
+                r.last_pre = pre;
+                changed.store(r);
+                S.stdp[e.tpos] = r;
+                cur = r.weight * receptor_gain(rec) * receptor_sign(rec);
+            }
+            if constexpr (!CODE_HERE) {
+                if (e.slot != kNone) {
+                    const uint32_t slot = e.slot;
+                    uint32_t window = S.tp_window, max_refs = S.tp_max_refs;
+                    float sim = S.tp_sim;
+                    if (!S.tp_uniform) {
+                        const NeuronParams p = S.np[x];
+                        window = p.window;
+                        sim = p.sim_thr;
+                        max_refs = p.max_refs;
+                    }
+                    if (CODE2) err |= code_observe2(S.code2[slot], T, 1, window, sim, max_refs);
+                    else err |= code_observeN(S.codeN[slot], T, 1, code_expected(S, slot), window, sim, max_refs);
+                    if (S.pre_count) S.pre_count[slot] += 1;
+                }
+            }
+        }
+        const float applied_current = cur;
+        f.cur[0] = comp == 0 ? f.cur[0] + applied_current : f.cur[0];
+        f.cur[1] = comp == 1 ? f.cur[1] + applied_current : f.cur[1];
+        f.cur[2] = comp == 2 ? f.cur[2] + applied_current : f.cur[2];
+        f.cur[3] = comp == 3 ? f.cur[3] + applied_current : f.cur[3];
+        f.taus_fires = (f.taus_fires & ~(3u << (2 * comp))) | (rec << (2 * comp));
+        // the membrane is not carried: after every event it is the sum of the compartments (zero
+        // after a reset), so fold_end recomputes the last one
+        const float membrane = f.cur[0] + f.cur[1] + f.cur[2] + f.cur[3];
+        if (membrane >= f.threshold) {
+            f.cur[0] = 0.0f;
+            f.cur[1] = 0.0f;
+            f.cur[2] = 0.0f;
+            f.cur[3] = 0.0f;
+            if ((f.taus_fires >> 8) == kFoldMaxFires) err |= kErrCounter;
+            if ((f.taus_fires >> 8) == 0) f.first_g = g;
+            f.taus_fires = ((f.taus_fires & ~0xFFu) | kResetTauCodes) + (1u << 8);
+            SNN_ATOMIC_OR(&S.fire_bits[g >> 5], 1u << (g & 31u));
+            S.fire_neuron[g] = x;
+            S.fire_mult[g] = 0;
+            if (S.fire_evt) S.fire_evt[g] = f.k + i;
+        }
+    }
+    f.k += kc;
+    return err;
+}
+
+// End of target x's tick: the neuron temporal pattern's k observations (it does not feed the
+// membrane, so they are applied together), then the state written back. Nothing is written for a
+// target without events. The synapses whose events came before one of this tick's fires are
+// brought up to every fire by the post-synaptic scan kernel (post_scan_record). fold_end_load
+// reads the target's stored dynamics and temporal header (so a thread can issue the loads of all
+// its targets before it applies any), fold_end_apply finishes.
+SNN_HD void fold_end_load(const Step& S, const FoldTarget& f, uint32_t x, NeuronDyn& d, TemporalHdr& h) {
+    if (f.k == 0) return;
+    d = S.dyn[x];
+    h = S.th[x];
+}
+
+SNN_HD uint32_t fold_end_apply(const Step& S, const FoldTarget& f, uint32_t x, uint32_t T, NeuronDyn d, TemporalHdr h) {
+    if (f.k == 0) return 0;
+    NeuronParams p;
+    if (S.tp_uniform) {
+        p.window = S.tp_window;
+        p.sim_thr = S.tp_sim;
+        p.max_refs = S.tp_max_refs;
+    } else {
+        p = S.np[x];
+    }
+    p.threshold = f.threshold;
+    const uint32_t err = temporal_observe_tick(h, &S.cand[x], &S.refs[x], T, f.k, p);
+    const uint32_t fires = f.taus_fires >> 8;
+    if (fires) {
+        S.fire_mult[f.first_g] = fires;
+        S.last_fired[x] = T;
+    }
+    d.cur[0] = f.cur[0];
+    d.cur[1] = f.cur[1];
+    d.cur[2] = f.cur[2];
+    d.cur[3] = f.cur[3];
+    d.membrane = f.cur[0] + f.cur[1] + f.cur[2] + f.cur[3];
+    d.last_tick = T;
+    d.spike_count += fires;
+    d.taus = f.taus_fires & 0xFFu;
+    S.dyn[x] = d;
+    S.th[x] = h;
+    return err;
+}
+
+SNN_HD uint32_t fold_end(const Step& S, const FoldTarget& f, uint32_t x, uint32_t T) {
+    NeuronDyn d{};
+    TemporalHdr h{};
+    fold_end_load(S, f, x, d, h);
+    return fold_end_apply(S, f, x, T, d, h);
+}
+
+// The sequential fold of target x's k events at tick T (ev is the target's own array, in any
+// order; it is sorted by g here, with networks of at most 8 entries when SMALL; narrow_g: every g of
+// the tick is below 2^27). Returns error flags.
+template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2, bool SMALL = false, typename E>
+SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32_t T, bool narrow_g) {
     if (k == 0) return 0;
-    sort_by_g(ev, k, S.seg_count[1] <= (1u << 27));
+    sort_by_g<SMALL>(ev, k, narrow_g);
 
     const NeuronParams p = S.np[x];
     NeuronDyn d = S.dyn[x];
@@ -460,7 +623,7 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
         const uint32_t g = e.gk >> 4;
         const uint32_t comp = (e.gk >> 2) & 3u;
         const uint32_t rec = e.gk & 3u;
-        float w = e.w;
+        float cur = e.w;  // the receptor current, formed in make_event
         if constexpr (WIDE) {
             if (STDP && e.tpos != kNone) {
                 StdpRec r = S.stdp[e.tpos];
@@ -475,7 +638,7 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
                 r.last_pre = pre;
                 changed.store(r);
                 S.stdp[e.tpos] = r;
-                w = r.weight;
+                cur = r.weight * receptor_gain(rec) * receptor_sign(rec);
             }
             if constexpr (!CODE_HERE) {
                 if (e.slot != kNone) {
@@ -487,7 +650,7 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
             }
         }
         // process_input (:175-212)
-        const float applied_current = w * receptor_gain(rec) * receptor_sign(rec);
+        const float applied_current = cur;
         // cur[comp] += applied, written as selects so the compartments stay in registers
         d.cur[0] = comp == 0 ? d.cur[0] + applied_current : d.cur[0];
         d.cur[1] = comp == 1 ? d.cur[1] + applied_current : d.cur[1];

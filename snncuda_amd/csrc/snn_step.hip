@@ -510,29 +510,232 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
     if (err) atomicOr(&S.ctl[1], err);
 }
 
-// LDS stage of one fold round; with the 4096-neuron partition's places (16 KB) and the per-target
-// cursors (16 KB) the fold uses all but 2 KB of the 160 KB of LDS a workgroup may hold
-constexpr uint32_t kStageBytes = 129024;
+// LDS of the 1024-thread fold: a 96 KB stage, two 16 KB count arrays (one per chunk in flight) and a
+// 16 KB table (scatter-workgroup run starts, or the sparse path's touched-target ranks)
+constexpr uint32_t kStageBytes = 98304;
 constexpr uint32_t kMaxPartition = 4096;  // neurons per partition at most (partition shift <= 12)
 constexpr int kSmallFold = 256;              // the small fold workgroup (sparse workloads)
 constexpr uint32_t kSmallStageBytes = 16384;  // its LDS stage: four workgroups per CU
 
-// One workgroup per target partition. A counting sort by target over the partition gives every
-// target's place. Dense partitions are folded in rounds of 1024 targets (one per thread), each round
-// staging only its own targets' events in LDS (global scratch if they do not fit), so the bucket's
-// 2-byte target indices are read once plus once per round and each event once. Partitions whose
-// events all fit the stage take one round over their touched targets only.
+// Dense partition in rounds of BLOCK targets, each round staging its own targets' events (in LDS,
+// or global scratch when they do not fit); off holds the partition's per-target counts. The
+// 256-thread fold's dense path, and the 1024-thread fold's when a scatter run exceeds its stage.
+template <int BLOCK, bool WIDE, bool STDP, bool CODE_HERE, bool CODE2, typename E>
+__device__ __forceinline__ void fold_rounds(const Step& S, E* lds_stage, uint32_t* aux, uint32_t* off, uint32_t c,
+                                            uint32_t start, uint32_t T) {
+    constexpr uint32_t kRound = BLOCK;
+    constexpr uint32_t kCap = (BLOCK == kPBlock ? kStageBytes : kSmallStageBytes) / sizeof(E);
+    constexpr uint32_t kScanPer = 1024 / BLOCK;
+    constexpr uint32_t kUnroll = WIDE ? 4 : 8;
+    constexpr uint32_t kNoTarget = 0xFFFFu;
+    const uint32_t PS = 1u << S.pshift;
+    const uint32_t p = blockIdx.x;
+    const E* src = (const E*)S.prec + start;
+    const uint16_t* tls = S.ptl + start;
+    const bool narrow_g = S.seg_count[1] <= (1u << 27);
+    uint32_t err = 0;
+    uint32_t run = 0;
+    for (uint32_t t0 = 0; t0 < PS; t0 += kScanPer * BLOCK) {
+        const uint32_t tb = t0 + threadIdx.x * kScanPer;
+        uint32_t v[kScanPer], sum = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kScanPer; ++q) {
+            v[q] = off[tb + q];
+            sum += v[q];
+        }
+        uint32_t tot;
+        uint32_t ex = run + block_excl_scan_n<BLOCK>(sum, &tot);
+#pragma unroll
+        for (uint32_t q = 0; q < kScanPer; ++q) {
+            off[tb + q] = ex;
+            ex += v[q];
+        }
+        run += tot;
+    }
+    if (threadIdx.x == 0) off[PS] = run;
+    __syncthreads();
+    uint32_t* fill = aux;
+    const uint32_t rounds = PS / kRound;
+    // one round staged at st: instantiated once on the LDS stage and once on global scratch, so the
+    // common case addresses the stage with LDS instructions rather than flat ones
+    auto round = [&](uint32_t r, uint32_t lo, E* st) {
+        const uint32_t t = r * kRound + threadIdx.x;
+        for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * BLOCK) {
+            uint32_t tv[kUnroll];
+            uint32_t ev[kUnroll][sizeof(E) / 4];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) tv[u] = i0 + u * BLOCK < c ? tls[i0 + u * BLOCK] : kNoTarget;
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u)
+                if (tv[u] / kRound == r) load_words(src + i0 + u * BLOCK, ev[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u)
+                if (tv[u] / kRound == r) store_words(st + atomicAdd(&fill[tv[u] % kRound], 1u), ev[u]);
+        }
+        __syncthreads();
+        const uint32_t x = (p << S.pshift) + t;
+        const uint32_t k = off[t + 1] - off[t];
+        if (x < S.n && k) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2, BLOCK == kPBlock>(S, st + (off[t] - lo), k, x, T, narrow_g);
+        __syncthreads();
+    };
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t lo = off[r * kRound], hi = off[(r + 1) * kRound];
+        fill[threadIdx.x] = off[r * kRound + threadIdx.x] - lo;
+        __syncthreads();
+        if (hi - lo <= kCap) round(r, lo, lds_stage);
+        else round(r, lo, (E*)S.stage + start + lo);
+    }
+    if (err) atomicOr(&S.ctl[1], err);
+}
+
+// Dense partition of the 1024-thread fold: the bucket in chunks of whole scatter-workgroup runs,
+// each counting-sorted by target into the LDS stage and folded into its targets' registers.
+//   runs: LDS table of the nwg run starts (relative to the partition's first event), then c; no
+//         run is longer than the stage
+//   off:  two arrays of PS + 1 counters (the chunk being placed and the next one)
+template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2, typename E>
+__device__ __forceinline__ void fold_chunks(const Step& S, E* lds_stage, uint32_t* runs, uint32_t* off, uint32_t c,
+                                            uint32_t start, uint32_t T) {
+    constexpr uint32_t BLOCK = kPBlock;
+    // a chunk of at most 8 records per thread (held in registers while the previous chunk is folded)
+    constexpr uint32_t R = (kStageBytes / sizeof(E) / BLOCK) < 8 ? (kStageBytes / sizeof(E) / BLOCK) : 8;
+    constexpr uint32_t kCap = R * BLOCK;
+    constexpr uint32_t TPT = kMaxPartition / BLOCK;  // targets per thread at most
+    constexpr uint32_t W = sizeof(E) / 4;
+    constexpr uint32_t kNoTarget = 0xFFFFu;
+    static_assert(kCap % BLOCK == 0, "the stage holds whole rounds of records");
+    const uint32_t PS = 1u << S.pshift;
+    const uint32_t tpt = PS / BLOCK;  // 1, 2 or 4 (partition shift 10 to 12)
+    const bool narrow_g = S.seg_count[1] <= (1u << 27);
+    const uint32_t p = blockIdx.x;
+    const uint32_t nwg = S.nwg;
+    const E* src = (const E*)S.prec + start;
+    const uint16_t* tls = S.ptl + start;
+    for (uint32_t t = threadIdx.x; t < 2 * (PS + 1); t += BLOCK) off[t] = 0;
+    // thread's targets: threadIdx.x + j * BLOCK, so the state loads and stores are coalesced
+    FoldTarget f[TPT];
+#pragma unroll
+    for (uint32_t j = 0; j < TPT; ++j) {
+        const uint32_t x = (p << S.pshift) + threadIdx.x + j * BLOCK;
+        f[j].k = 0;
+        if (j < tpt && x < S.n) fold_begin(S, f[j], x, T);
+    }
+    __syncthreads();
+    // End of the chunk that starts at the run start lo: the last run start within lo + kCap (no run
+    // exceeds kCap, so it is past lo). runs[0] = 0 <= lo and runs[nwg] = c keep the search in range.
+    auto chunk_end = [&](uint32_t lo) -> uint32_t {
+        uint32_t a = 0, b = nwg + 1;
+        while (a < b) {
+            const uint32_t m = (a + b) >> 1;
+            if (runs[m] <= lo + kCap) a = m + 1; else b = m;
+        }
+        return runs[a - 1];
+    };
+    uint32_t tv[R];
+    uint32_t ev[R][W];
+    // positions a + threadIdx.x + u * BLOCK (u < R) below b into registers
+    auto load = [&](uint32_t a, uint32_t b) {
+#pragma unroll
+        for (uint32_t u = 0; u < R; ++u) {
+            const uint32_t i = a + threadIdx.x + u * BLOCK;
+            tv[u] = i < b ? tls[i] : kNoTarget;
+            if (i < b) load_words(src + i, ev[u]);
+        }
+    };
+    uint32_t err = 0;
+    uint32_t* oc = off;           // counts, then places, of the current chunk
+    uint32_t* on = off + PS + 1;  // the next chunk's counts
+    uint32_t lo = 0;
+    uint32_t hi = chunk_end(0);
+    load(lo, hi);
+    while (lo < c) {
+#pragma unroll
+        for (uint32_t u = 0; u < R; ++u)
+            if (tv[u] != kNoTarget) atomicAdd(&oc[tv[u]], 1u);
+        __syncthreads();
+        {
+            // exclusive scan of the chunk's counts, tpt consecutive counters per thread
+            uint32_t v[TPT], sum = 0;
+            const uint32_t tb = threadIdx.x * tpt;
+#pragma unroll
+            for (uint32_t q = 0; q < TPT; ++q) {
+                v[q] = q < tpt ? oc[tb + q] : 0u;
+                sum += v[q];
+            }
+            uint32_t tot;
+            uint32_t ex = block_excl_scan_n<BLOCK>(sum, &tot);
+#pragma unroll
+            for (uint32_t q = 0; q < TPT; ++q) {
+                if (q < tpt) oc[tb + q] = ex;
+                ex += v[q];
+            }
+        }
+        __syncthreads();
+        // place the chunk, zero the next chunk's counters and start loading it, then fold this one
+#pragma unroll
+        for (uint32_t u = 0; u < R; ++u)
+            if (tv[u] != kNoTarget) store_words(lds_stage + atomicAdd(&oc[tv[u]], 1u), ev[u]);
+        for (uint32_t t = threadIdx.x; t < PS + 1; t += BLOCK) on[t] = 0;
+        const uint32_t nlo = hi;
+        const uint32_t nhi = nlo < c ? chunk_end(nlo) : nlo;
+        load(nlo, nhi);
+        __syncthreads();
+        // after the placement oc[t] is the end of target t's events, oc[t - 1] their start
+#pragma unroll
+        for (uint32_t j = 0; j < TPT; ++j) {
+            const uint32_t t = threadIdx.x + j * BLOCK;
+            const uint32_t x = (p << S.pshift) + t;
+            if (j < tpt && x < S.n) {
+                const uint32_t e = oc[t], s = t ? oc[t - 1] : 0u;
+                if (e > s) err |= fold_piece<WIDE, STDP, CODE_HERE, CODE2>(S, f[j], lds_stage + s, e - s, x, T, narrow_g);
+            }
+        }
+        uint32_t* const t = oc;
+        oc = on;
+        on = t;
+        lo = nlo;
+        hi = nhi;
+    }
+    // the stored state of all the thread's targets is loaded before any is finished
+    NeuronDyn d[TPT];
+    TemporalHdr h[TPT];
+#pragma unroll
+    for (uint32_t j = 0; j < TPT; ++j) {
+        const uint32_t x = (p << S.pshift) + threadIdx.x + j * BLOCK;
+        if (j < tpt && x < S.n) fold_end_load(S, f[j], x, d[j], h[j]);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < TPT; ++j) {
+        const uint32_t x = (p << S.pshift) + threadIdx.x + j * BLOCK;
+        if (j < tpt && x < S.n) err |= fold_end_apply(S, f[j], x, T, d[j], h[j]);
+    }
+    if (err) atomicOr(&S.ctl[1], err);
+}
+// One workgroup per target partition.
+//
+// Dense partitions (1024-thread fold): the bucket is read once, in chunks of whole scatter-workgroup
+// runs. The scatter workgroups own contiguous ranges of g in workgroup order, so a chunk holds, for
+// every target, events of larger g than the chunks before it. Each chunk is counting-sorted by
+// target into the LDS stage (a run larger than the stage goes through global scratch) and every
+// thread folds the chunk's events of its targets (PS / 1024 of them, state in registers across the
+// chunks). The next chunk's records are loaded while the current one is folded.
+//
+// Partitions whose events all fit the stage: one counting sort over the partition and one round
+// over the touched targets only. The 256-thread fold folds dense partitions in rounds of 256
+// targets, each round staging its own targets' events.
 template <int BLOCK, bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
 __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
     typedef typename EntryOf<WIDE, CODE_HERE>::type E;
     constexpr uint32_t kRound = BLOCK;  // targets folded per round (one per thread)
     constexpr uint32_t kCap = (BLOCK == kPBlock ? kStageBytes : kSmallStageBytes) / sizeof(E);
     __shared__ E lds_stage[kCap];
-    // dense rounds: each target's next stage slot (kRound entries); sparse partitions: per touched
-    // target (by rank, at most kCap) its index | next stage slot << 16
+    // 1024-thread fold: run starts of the scatter workgroups (dense) or per touched target (by rank,
+    // at most kCap) its index | next stage slot << 16 (sparse); 256-thread fold: also each round
+    // target's next stage slot
     constexpr uint32_t kAux = BLOCK == kPBlock ? kMaxPartition : (kCap > kRound ? kCap : kRound);
     __shared__ uint32_t aux[kAux];
-    extern __shared__ uint32_t off[];  // PS + 1: event counts per target, then their places
+    // PS + 1 event counts per target, then their places (the 1024-thread fold: two such arrays)
+    extern __shared__ uint32_t off[];
     const uint32_t PS = 1u << S.pshift;
     // counters per thread in the scans over the partition's targets: 1,024 per block scan (the
     // partition holds 1,024 to 4,096 targets)
@@ -544,12 +747,30 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
     const uint32_t c = end - start;
     const E* src = (const E*)S.prec + start;
     const uint16_t* tls = S.ptl + start;
+    constexpr uint32_t kNoTarget = 0xFFFFu;
+    if constexpr (BLOCK == kPBlock) {
+        if (c > kCap) {
+            // chunks of whole scatter-workgroup runs, unless one run alone exceeds the stage (a burst
+            // concentrated on few partitions): then rounds of 1024 targets as below
+            const uint32_t nwg = S.nwg;
+            bool big = false;
+            for (uint32_t w = threadIdx.x; w < nwg; w += BLOCK) {
+                aux[w] = S.hist[(uint64_t)p * nwg + w] - start;
+                const uint32_t nxt = w + 1 < nwg ? S.hist[(uint64_t)p * nwg + w + 1] - start : c;
+                big |= nxt - aux[w] > kCap;
+            }
+            if (threadIdx.x == 0) aux[nwg] = c;
+            if (!__syncthreads_or(big)) {
+                fold_chunks<WIDE, STDP, CODE_HERE, CODE2>(S, lds_stage, aux, off, c, start, T);
+                return;
+            }
+        }
+    }
     for (uint32_t t = threadIdx.x; t < PS; t += blockDim.x) off[t] = 0;
     __syncthreads();
     // the passes over the bucket load kUnroll records per thread before using them, so that many
     // loads are in flight per wave
     constexpr uint32_t kUnroll = WIDE ? 4 : 8;
-    constexpr uint32_t kNoTarget = 0xFFFFu;
     for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * BLOCK) {
         uint32_t tv[kUnroll];
 #pragma unroll
@@ -560,6 +781,7 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
     }
     __syncthreads();
     uint32_t err = 0;
+    const bool narrow_g = S.seg_count[1] <= (1u << 27);
     if (c <= kCap) {
         // Sparse partition: every event fits the stage at once, so one round folds the touched
         // targets only, one per thread. One scan gives both the targets' places (low 16 bits) and
@@ -604,63 +826,12 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
             const uint32_t t = aux[j] & 0xFFFFu;
             const uint32_t x = (p << S.pshift) + t;
             const uint32_t at = off[t] & 0xFFFFu;
-            if (x < S.n) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2>(S, lds_stage + at, (off[t + 1] & 0xFFFFu) - at, x, T);
+            if (x < S.n) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2>(S, lds_stage + at, (off[t + 1] & 0xFFFFu) - at, x, T, narrow_g);
         }
         if (err) atomicOr(&S.ctl[1], err);
         return;
     }
-    uint32_t run = 0;
-    for (uint32_t t0 = 0; t0 < PS; t0 += kScanPer * BLOCK) {
-        const uint32_t tb = t0 + threadIdx.x * kScanPer;
-        uint32_t v[kScanPer], sum = 0;
-#pragma unroll
-        for (uint32_t q = 0; q < kScanPer; ++q) {
-            v[q] = off[tb + q];
-            sum += v[q];
-        }
-        uint32_t tot;
-        uint32_t ex = run + block_excl_scan_n<BLOCK>(sum, &tot);
-#pragma unroll
-        for (uint32_t q = 0; q < kScanPer; ++q) {
-            off[tb + q] = ex;
-            ex += v[q];
-        }
-        run += tot;
-    }
-    if (threadIdx.x == 0) off[PS] = run;
-    __syncthreads();
-    uint32_t* fill = aux;
-    const uint32_t rounds = PS / kRound;
-    // one round staged at st: instantiated once on the LDS stage and once on global scratch, so the
-    // common case addresses the stage with LDS instructions rather than flat ones
-    auto round = [&](uint32_t r, uint32_t lo, E* st) {
-        const uint32_t t = r * kRound + threadIdx.x;
-        for (uint32_t i0 = threadIdx.x; i0 < c; i0 += kUnroll * BLOCK) {
-            uint32_t tv[kUnroll];
-            uint32_t ev[kUnroll][sizeof(E) / 4];
-#pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) tv[u] = i0 + u * BLOCK < c ? tls[i0 + u * BLOCK] : kNoTarget;
-#pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u)
-                if (tv[u] / kRound == r) load_words(src + i0 + u * BLOCK, ev[u]);
-#pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u)
-                if (tv[u] / kRound == r) store_words(st + atomicAdd(&fill[tv[u] % kRound], 1u), ev[u]);
-        }
-        __syncthreads();
-        const uint32_t x = (p << S.pshift) + t;
-        const uint32_t k = off[t + 1] - off[t];
-        if (x < S.n && k) err |= fold_events<WIDE, STDP, CODE_HERE, CODE2>(S, st + (off[t] - lo), k, x, T);
-        __syncthreads();
-    };
-    for (uint32_t r = 0; r < rounds; ++r) {
-        const uint32_t lo = off[r * kRound], hi = off[(r + 1) * kRound];
-        fill[threadIdx.x] = off[r * kRound + threadIdx.x] - lo;
-        __syncthreads();
-        if (hi - lo <= kCap) round(r, lo, lds_stage);
-        else round(r, lo, (E*)S.stage + start + lo);
-    }
-    if (err) atomicOr(&S.ctl[1], err);
+    fold_rounds<BLOCK, WIDE, STDP, CODE_HERE, CODE2>(S, lds_stage, aux, off, c, start, T);
 }
 
 // rank-ordered fire list of tick T into the fire ring (and the raster)
@@ -1167,6 +1338,7 @@ struct PendingInject {
     uint64_t seq;
 };
 
+constexpr int kProfileSelect = 0x40000000;  // snn_net_set_profiling: time only the kernels in the mask
 enum KernelId { KPlan, KSegments, KScanSeg, KCount, KScanTgt, KScatter, KFold, KScanFire, KFireWrite, KPost, KAdvance, KCount_ };
 const char* kKernelNames[KCount_] = {"plan", "segments", "scan_segments", "count", "scan_partitions", "scatter",
                                      "fold", "scan_fires", "fire_write", "post", "advance"};
@@ -1200,6 +1372,8 @@ struct snn_net {
     uint64_t n_tpos = 0;
     // profiling
     bool profiling = false;
+    uint32_t prof_mask = ~0u;           // kernels timed (bit = KernelId)
+    std::vector<hipEvent_t> event_pool;
     double kms[KCount_] = {0};
     uint64_t klaunch[KCount_] = {0};
     // launch schedule (results are identical under every schedule)
@@ -1216,6 +1390,7 @@ struct snn_net {
     ~snn_net() {
         if (device >= 0) (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto e : event_pool) (void)hipEventDestroy(e);
         for (auto& b : run_owned) (void)hipFree(b.p);
         for (auto& b : owned) (void)hipFree(b.p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -1265,12 +1440,13 @@ void launch_scatter(snn_net* net) {
 
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
 void launch_fold(snn_net* net) {
+    // per-target counters: one array for the 256-thread fold, two for the 1024-thread fold's chunks
     const size_t lds = ((1u << net->S.pshift) + 1) * 4;
     if (net->fold_threads == (uint32_t)kSmallFold)
         hipLaunchKernelGGL((k_pfold<kSmallFold, WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kSmallFold),
                            lds, net->stream, net->S);
     else
-        hipLaunchKernelGGL((k_pfold<kPBlock, WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kPBlock), lds,
+        hipLaunchKernelGGL((k_pfold<kPBlock, WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kPBlock), 2 * lds,
                            net->stream, net->S);
 }
 
@@ -1337,24 +1513,37 @@ void dispatch_fold(snn_net* net) {
 #undef SNN_FOLD
 }
 
+// HIP events around the profiled kernels (all, or those selected by the profiling mask), taken
+// from a pool owned by the handle so that a timed run creates none.
 struct Timer {
     snn_net* net;
     bool enabled = true;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     int cur = -1;
     hipEvent_t a = nullptr;
+    hipEvent_t take() {
+        if (net->event_pool.empty()) {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "hipEventCreate");
+            return e;
+        }
+        hipEvent_t e = net->event_pool.back();
+        net->event_pool.pop_back();
+        return e;
+    }
     void begin(int id) {
-        if (!enabled || !net->profiling) return;
+        cur = -1;
+        if (!enabled || !net->profiling || !((net->prof_mask >> id) & 1u)) return;
         cur = id;
-        check(hipEventCreate(&a), "hipEventCreate");
+        a = take();
         check(hipEventRecord(a, net->stream), "hipEventRecord");
     }
     void end() {
-        if (!enabled || !net->profiling) return;
-        hipEvent_t b;
-        check(hipEventCreate(&b), "hipEventCreate");
+        if (cur < 0) return;
+        hipEvent_t b = take();
         check(hipEventRecord(b, net->stream), "hipEventRecord");
         marks.push_back({cur, {a, b}});
+        cur = -1;
     }
     void collect() {
         for (auto& m : marks) {
@@ -1362,8 +1551,8 @@ struct Timer {
             check(hipEventElapsedTime(&ms, m.second.first, m.second.second), "hipEventElapsedTime");
             net->kms[m.first] += ms;
             net->klaunch[m.first] += 1;
-            (void)hipEventDestroy(m.second.first);
-            (void)hipEventDestroy(m.second.second);
+            net->event_pool.push_back(m.second.first);
+            net->event_pool.push_back(m.second.second);
         }
         marks.clear();
     }
@@ -2539,6 +2728,7 @@ int snn_net_read_plasticity(snn_net* net, uint64_t* ltp_updates, uint64_t* ltd_u
 int snn_net_set_profiling(snn_net* net, int enabled) {
     if (!net) return fail(SNN_EINVAL, "null handle");
     net->profiling = enabled != 0;
+    net->prof_mask = (enabled & kProfileSelect) ? (uint32_t)(enabled & ~kProfileSelect) : ~0u;
     for (int k = 0; k < KCount_; ++k) {
         net->kms[k] = 0;
         net->klaunch[k] = 0;

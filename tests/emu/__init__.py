@@ -108,12 +108,13 @@ class Emu:
         return o
 
 
-def sort_by_g(words, narrow_g):
+def sort_by_g(words, narrow_g, small=False):
     """The device's per-target sort, run on the CPU, of entries given as rows of 2, 3 or 4 uint32
     words (8-, 12- or 16-byte entries, gk first); returns the sorted rows.
-    narrow_g: every g is below 2^27 (the key-and-place networks may be used)."""
+    narrow_g: every g is below 2^27 (the key-and-place networks may be used).
+    small: the chunked fold's variant (networks of at most 8 entries)."""
     a = np.ascontiguousarray(words, np.uint32).copy()
-    lib().emu_sort_by_g(_p(a), a.shape[0], int(narrow_g), a.shape[1])
+    lib().emu_sort_by_g(_p(a), a.shape[0], int(narrow_g) | (2 if small else 0), a.shape[1])
     return a
 
 

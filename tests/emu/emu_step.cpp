@@ -84,6 +84,35 @@ uint32_t deliver(Emu& m, const std::vector<std::pair<uint32_t, uint32_t>>& order
         const uint32_t q = order[k].first;
         err |= make_event<WIDE, CH, C2>(m.S, m.S.seg_v0[q], m.S.seg_off[q], m.S.seg_mult[q], order[k].second, T, xs[k], recs[k]);
     }
+    const bool narrow = m.S.seg_count[1] <= (1u << 27);
+    if (m.rng() & 1u) {
+        // the dense fold's chunks: the events in g order cut at random places (whole scatter runs
+        // are contiguous g ranges), each chunk shuffled, its events grouped by target and folded
+        // into the targets' running state (fold_begin / fold_piece / fold_end)
+        std::vector<size_t> idx(recs.size());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return (recs[a].gk >> 4) < (recs[b].gk >> 4); });
+        std::vector<FoldTarget> f(m.S.n);
+        for (uint32_t x = 0; x < m.S.n; ++x) fold_begin(m.S, f[x], x, T);
+        size_t lo = 0;
+        while (lo < idx.size()) {
+            const size_t hi = std::min(idx.size(), lo + 1 + (size_t)(m.rng() % 64));
+            std::vector<size_t> ch(idx.begin() + lo, idx.begin() + hi);
+            std::shuffle(ch.begin(), ch.end(), m.rng);
+            std::stable_sort(ch.begin(), ch.end(), [&](size_t a, size_t b) { return xs[a] < xs[b]; });
+            std::vector<E> piece;
+            for (size_t a = 0; a < ch.size();) {
+                size_t b = a;
+                piece.clear();
+                while (b < ch.size() && xs[ch[b]] == xs[ch[a]]) piece.push_back(recs[ch[b++]]);
+                err |= fold_piece<WIDE, ST, CH, C2>(m.S, f[xs[ch[a]]], piece.data(), (uint32_t)piece.size(), xs[ch[a]], T, narrow);
+                a = b;
+            }
+            lo = hi;
+        }
+        for (uint32_t x = 0; x < m.S.n; ++x) err |= fold_end(m.S, f[x], x, T);
+        return err;
+    }
     std::vector<uint32_t> cnt(m.S.n + 1, 0);
     for (uint32_t x : xs) cnt[x + 1]++;
     for (uint32_t i = 0; i < m.S.n; ++i) cnt[i + 1] += cnt[i];
@@ -91,7 +120,7 @@ uint32_t deliver(Emu& m, const std::vector<std::pair<uint32_t, uint32_t>>& order
     std::vector<E> by(recs.size());
     for (size_t k = 0; k < recs.size(); ++k) by[fill[xs[k]]++] = recs[k];
     for (uint32_t x = 0; x < m.S.n; ++x)
-        err |= fold_events<WIDE, ST, CH, C2>(m.S, by.data() + cnt[x], cnt[x + 1] - cnt[x], x, T);
+        err |= fold_events<WIDE, ST, CH, C2>(m.S, by.data() + cnt[x], cnt[x + 1] - cnt[x], x, T, narrow);
     return err;
 }
 
@@ -216,12 +245,14 @@ std::vector<float> decay_table() {
 
 }  // namespace
 
-// the per-target sort on E entries held as 32-bit words
+// the per-target sort on E entries held as 32-bit words (small: the 8-entry networks of the
+// chunked fold, shell sort past them)
 template <typename E>
-void sort_words(uint32_t* words, uint32_t k, bool narrow_g) {
+void sort_words(uint32_t* words, uint32_t k, bool narrow_g, bool small) {
     std::vector<E> ev(k);
     std::memcpy(ev.data(), words, k * sizeof(E));
-    sort_by_g(ev.data(), k, narrow_g);
+    if (small) sort_by_g<true>(ev.data(), k, narrow_g);
+    else sort_by_g(ev.data(), k, narrow_g);
     std::memcpy(words, ev.data(), k * sizeof(E));
 }
 
@@ -442,10 +473,12 @@ void emu_synapses(void* p, float* w, uint64_t* pre, uint64_t* cmatch, uint64_t* 
 }
 
 // the per-target sort alone, on k entries of `width` 32-bit words (gk first) in place
+// narrow_g: bit 0 every g below 2^27, bit 1 the small networks
 void emu_sort_by_g(uint32_t* words, uint32_t k, uint32_t narrow_g, uint32_t width) {
-    if (width == 2) sort_words<Entry8>(words, k, narrow_g != 0);
-    else if (width == 3) sort_words<Entry12>(words, k, narrow_g != 0);
-    else sort_words<Entry16>(words, k, narrow_g != 0);
+    const bool narrow = narrow_g & 1u, small = narrow_g & 2u;
+    if (width == 2) sort_words<Entry8>(words, k, narrow, small);
+    else if (width == 3) sort_words<Entry12>(words, k, narrow, small);
+    else sort_words<Entry16>(words, k, narrow, small);
 }
 
 // One neuron's temporal pattern driven by m ticks with times[i] observations at ticks[i], once

@@ -30,9 +30,10 @@ def test_capacity_overflow_is_an_error_not_a_drop():
     assert m.flags & 1  # events per tick
 
 
+@pytest.mark.parametrize("small", [False, True])
 @pytest.mark.parametrize("width", [2, 3, 4])
 @pytest.mark.parametrize("narrow_g", [False, True])
-def test_per_target_sort_orders_by_g_at_every_size(narrow_g, width):
+def test_per_target_sort_orders_by_g_at_every_size(narrow_g, width, small):
     # 8-, 12- and 16-byte entries at sizes on both sides of the sorting networks (8, 16, 24) and the
     # shell sort past them; g is unique within a tick, the low 4 bits carry compartment and
     # receptor, and every other word must travel with its event. The largest g a tick can hold is
@@ -47,7 +48,7 @@ def test_per_target_sort_orders_by_g_at_every_size(narrow_g, width):
                 g[0] = gmax - 1
             rows = rng.integers(0, 1 << 32, size=(k, width), dtype=np.uint64).astype(np.uint32)
             rows[:, 0] = (g << 4) | rng.integers(0, 16, size=k, dtype=np.uint32)
-            out = sort_by_g(rows, narrow_g)
+            out = sort_by_g(rows, narrow_g, small)
             assert np.array_equal(out, rows[np.argsort(g, kind="stable")]), k
 
 
