@@ -112,6 +112,16 @@ int snn_net_inject(snn_net* net, const uint32_t* neuron, const uint64_t* tick, c
  * and including inclusive_tick. Synchronous; the per-tick loop itself never waits on the host. */
 int snn_net_run_until(snn_net* net, uint64_t inclusive_tick);
 
+/* No counterpart in the reference. snn_net_run_until for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * ticks and results, with the run segment's inject tables and per-tick counters staged in a fixed
+ * 4 MiB device array that each device holds until the process ends, instead of buffers allocated
+ * and freed per call; no HIP graph replay, autotuning or profiling events inside the call. A segment
+ * that needs more staging than the array holds (about 262,000 ticks without injects, 12 bytes per
+ * pending inject) fails with SNN_EINVAL and changes nothing. All calls on one device share its
+ * array: calls that overlap on one device race. */
+int snn_net_run_until_static(snn_net* net, uint64_t inclusive_tick);
+
 /* delivered_spike_count / fired_spike_count (NetworkPropagator.h:27-28) and the next tick. */
 int snn_net_counts(snn_net* net, uint64_t* delivered, uint64_t* fired, uint64_t* next_tick);
 

@@ -95,6 +95,7 @@ def _load() -> ctypes.CDLL:
     L.snn_net_destroy.argtypes = [P]
     L.snn_net_inject.argtypes = [P, P, P, P, U64]
     L.snn_net_run_until.argtypes = [P, U64]
+    L.snn_net_run_until_static.argtypes = [P, U64]
     L.snn_net_counts.argtypes = [P, P, P, P]
     L.snn_net_read_tick_counts.argtypes = [P, U64, U64, P, P]
     L.snn_net_read_raster.argtypes = [P, P, P, U64]
@@ -124,7 +125,7 @@ def _load() -> ctypes.CDLL:
     L.snn_batch_net_counts.argtypes = [P, P, P]
     L.snn_net_set_profiling.argtypes = [P, I]
     L.snn_net_kernel_times.argtypes = [P, P, P, I, ctypes.c_char_p, U64]
-    for name in ("snn_net_create", "snn_net_create_ei", "snn_net_destroy", "snn_net_inject", "snn_net_run_until", "snn_net_counts",
+    for name in ("snn_net_create", "snn_net_create_ei", "snn_net_destroy", "snn_net_inject", "snn_net_run_until", "snn_net_run_until_static", "snn_net_counts",
                  "snn_net_read_tick_counts", "snn_net_read_raster", "snn_net_read_neurons",
                  "snn_net_read_synapses", "snn_net_read_plasticity", "snn_net_set_profiling",
                  "snn_net_kernel_times", "snn_net_get_schedule", "snn_net_set_schedule", "snn_net_autotune",
@@ -237,6 +238,11 @@ class Network:
 
     def run_until(self, inclusive_tick: int) -> None:
         _check(lib().snn_net_run_until(self._h, int(inclusive_tick)))
+
+    def run_until_static(self, inclusive_tick: int) -> None:
+        """run_until without device allocations: the segment's inject tables and counters are staged
+        in the library's fixed per-device array (snn_net_run_until_static)."""
+        _check(lib().snn_net_run_until_static(self._h, int(inclusive_tick)))
 
     def run_commands(self, commands: Sequence) -> None:
         """Replay (kind, neuron, tick, weight) commands: kind 0 inject, 1 run_until. Consecutive

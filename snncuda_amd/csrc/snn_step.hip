@@ -2479,7 +2479,40 @@ int snn_net_inject(snn_net* net, const uint32_t* neuron, const uint64_t* tick, c
     return SNN_OK;
 }
 
-int snn_net_run_until(snn_net* net, uint64_t inclusive_tick) {
+}  // extern "C"
+
+namespace {
+
+// Staging of snn_net_run_until_static: the run segment's inject tables and per-tick counters in a
+// fixed device array instead of buffers allocated per call. Every call on a device shares that
+// device's copy of the array (calls that overlap on one device race); it holds the last call's
+// contents, all of which a call overwrites before its kernels read them.
+constexpr size_t kStaticScratchBytes = 4u << 20;
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+// Device address of `scratch` on each device, looked up once per device with that device current
+// (the host-side name is only a stand-in for it).
+uint8_t* scratch_on(int device) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex m;
+    static uint8_t* ptr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) throw std::invalid_argument("device ordinal out of range");
+    std::lock_guard<std::mutex> lock(m);
+    if (!ptr[device]) {
+        check(hipSetDevice(device), "hipSetDevice");
+        void* p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        ptr[device] = (uint8_t*)p;
+    }
+    return ptr[device];
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// run_until with the segment's buffers allocated per call (static_staging false), or staged in
+// `scratch` with nothing allocated or freed on the device (true: stream launches only, no HIP
+// graph, no autotuning, no profiling events)
+int run_until_impl(snn_net* net, uint64_t inclusive_tick, bool static_staging) {
     if (!net) return fail(SNN_EINVAL, "null handle");
     if (inclusive_tick < net->next_tick) return SNN_OK;
     return guarded([&]() -> int {
@@ -2495,6 +2528,16 @@ int snn_net_run_until(snn_net* net, uint64_t inclusive_tick) {
             if (a.stage != b.stage) return a.stage < b.stage;
             return a.seq < b.seq;
         });
+        // staging layout of snn_net_run_until_static, checked before anything changes
+        const size_t ni = now.size();
+        const size_t o_w = align256(ni * 4), o_stage = o_w + align256(ni * 4), o_ptr = o_stage + align256(ni * 4);
+        const size_t o_del = o_ptr + align256((nt + 1) * 8), o_fir = o_del + align256(nt * 4);
+        const size_t need = o_fir + align256(nt * 4);
+        if (static_staging && need > kStaticScratchBytes)
+            return fail(SNN_EINVAL, "snn_net_run_until_static: the run segment (" + std::to_string(nt) + " ticks, " +
+                                        std::to_string(ni) + " injects) needs " + std::to_string(need) +
+                                        " bytes of staging; the static scratch holds " +
+                                        std::to_string(kStaticScratchBytes) + " (use shorter segments)");
         net->pending.swap(later);
         if (net->record) net->inj_log.insert(net->inj_log.end(), now.begin(), now.end());
         std::vector<uint32_t> itgt(now.size()), istage(now.size());
@@ -2507,23 +2550,44 @@ int snn_net_run_until(snn_net* net, uint64_t inclusive_tick) {
             iptr[now[i].tick - t0 + 1]++;
         }
         for (uint64_t t = 0; t < nt; ++t) iptr[t + 1] += iptr[t];
-        for (auto& b : net->run_owned) (void)hipFree(b.p);
-        net->run_owned.clear();
         Step& S = net->S;
-        S.inj_tgt = dev_upload(net->run_owned, itgt);
-        S.inj_w = dev_upload(net->run_owned, iw);
-        S.inj_stage = dev_upload(net->run_owned, istage);
-        S.inj_ptr = dev_upload(net->run_owned, iptr);
+        if (static_staging) {
+            uint8_t* base = scratch_on(net->device);
+            check(hipSetDevice(net->device), "hipSetDevice");
+            if (ni) {
+                check(hipMemcpy(base, itgt.data(), ni * 4, hipMemcpyHostToDevice), "stage injects");
+                check(hipMemcpy(base + o_w, iw.data(), ni * 4, hipMemcpyHostToDevice), "stage injects");
+                check(hipMemcpy(base + o_stage, istage.data(), ni * 4, hipMemcpyHostToDevice), "stage injects");
+            }
+            check(hipMemcpy(base + o_ptr, iptr.data(), (nt + 1) * 8, hipMemcpyHostToDevice), "stage injects");
+            S.inj_tgt = (const uint32_t*)base;
+            S.inj_w = (const float*)(base + o_w);
+            S.inj_stage = (const uint32_t*)(base + o_stage);
+            S.inj_ptr = (const uint64_t*)(base + o_ptr);
+            // every tick of the segment writes its two counters (k_advance), so they need no clearing
+            S.tick_delivered = (uint32_t*)(base + o_del);
+            S.tick_fired = (uint32_t*)(base + o_fir);
+        } else {
+            for (auto& b : net->run_owned) (void)hipFree(b.p);
+            net->run_owned.clear();
+            S.inj_tgt = dev_upload(net->run_owned, itgt);
+            S.inj_w = dev_upload(net->run_owned, iw);
+            S.inj_stage = dev_upload(net->run_owned, istage);
+            S.inj_ptr = dev_upload(net->run_owned, iptr);
+            S.tick_delivered = dev_alloc<uint32_t>(net->run_owned, nt);
+            S.tick_fired = dev_alloc<uint32_t>(net->run_owned, nt);
+        }
         S.inj_t0 = (uint32_t)t0;
         S.inj_nticks = (uint32_t)nt;
-        S.tick_delivered = dev_alloc<uint32_t>(net->run_owned, nt);
-        S.tick_fired = dev_alloc<uint32_t>(net->run_owned, nt);
         {
             const uint32_t tick32 = (uint32_t)t0;
             check(hipMemcpy(S.ctl, &tick32, 4, hipMemcpyHostToDevice), "set tick");
         }
         Timer tm{net};
-        if (net->profiling) {
+        if (static_staging) {
+            tm.enabled = false;
+            for (uint64_t t = 0; t < nt; ++t) launch_tick(net, (uint32_t)t0, tm);
+        } else if (net->profiling) {
             for (uint64_t t = 0; t < nt; ++t) {
                 launch_tick(net, (uint32_t)t0, tm);
                 if (tm.marks.size() > 4096) {
@@ -2565,6 +2629,14 @@ int snn_net_run_until(snn_net* net, uint64_t inclusive_tick) {
         return SNN_OK;
     });
 }
+
+}  // namespace
+
+extern "C" {
+
+int snn_net_run_until(snn_net* net, uint64_t inclusive_tick) { return run_until_impl(net, inclusive_tick, false); }
+
+int snn_net_run_until_static(snn_net* net, uint64_t inclusive_tick) { return run_until_impl(net, inclusive_tick, true); }
 
 int snn_net_counts(snn_net* net, uint64_t* delivered, uint64_t* fired, uint64_t* next_tick) {
     if (!net) return fail(SNN_EINVAL, "null handle");
