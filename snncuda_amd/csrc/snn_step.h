@@ -138,7 +138,14 @@ struct Step {
     uint32_t* seg_mult;
     // target partitions: partition p holds neurons [p << pshift, (p+1) << pshift)
     uint32_t pshift, nparts, nwg;
-    uint32_t* hist;              // [nparts][nwg] per-workgroup event counts, then their bucket bases
+    uint32_t* hist;              // [nparts][nwg] per-workgroup event counts, then their offsets within
+                                 // the partition's bucket
+    uint32_t* ptotal;            // [nparts] events per partition (bucket p starts at the sum before it)
+    uint32_t* partials;          // block partial sums of the two-kernel scans (kScanBlocks)
+    // the tick being launched (kNone: read ctl[0], as HIP graph replays do), and whether the first
+    // kernel of the tick also finishes the previous tick's counters (stream launches)
+    uint32_t tick;
+    uint32_t fused_advance;
     // Code patterns kept per source neuron (code_lazy): with one code offset per synapse and the
     // same window / similarity / max_refs on every target, a synapse's code pattern is a function of
     // its source's fire ticks shifted by the synapse's lag, so one record per source, advanced when a
@@ -152,7 +159,7 @@ struct Step {
     uint32_t* fire_bits;
     uint32_t* fire_neuron;
     uint32_t* fire_mult;
-    uint32_t* fire_word_off;
+    uint32_t* fire_word_off;     // the CPU model's fire ranks (the device ranks fires in k_fire_write)
     // device state
     uint32_t* ctl;               // [0] tick, [1] error flags, [2] fires this tick, [3] bit 0: raster full
     uint64_t* totals;            // [0] delivered, [1] fired, [2] fire-log head, [3] raster fill

@@ -56,6 +56,9 @@ constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kScanBlocks = 1024;
 constexpr int kPBlock = 1024;  // partition count / scatter workgroups: 16 waves per CU
+// scatter workgroups at most: k_hscan scans four counts per thread of a 256-thread block, and the
+// fold's LDS table holds every workgroup's run start
+constexpr uint32_t kMaxScatterWgs = 1024;
 
 // ------------------------------------------------------------------------------------ scans
 
@@ -196,63 +199,116 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(Src src, const uint32_t* p
 
 // ------------------------------------------------------------------------------------ step kernels
 
-__global__ void __launch_bounds__(1024) k_plan(Step S) {
-    const uint32_t T = S.ctl[0];
+__device__ __forceinline__ uint32_t tick_of(const Step& S) { return S.tick != kNone ? S.tick : S.ctl[0]; }
+
+// Counters of finished tick T (its delivered / fired counts, the fire-log head, the raster fill)
+// and the device tick. One thread.
+__device__ void advance_tick(const Step& S, uint32_t T) {
+    const uint32_t F = S.ctl[2];
+    const uint32_t D = S.seg_count[1];
+    S.tick_delivered[T - S.inj_t0] = D;
+    S.tick_fired[T - S.inj_t0] = F;
+    S.totals[0] += D;
+    S.totals[1] += F;
+    S.totals[2] += S.tick_fcount[T % (S.lmax + 1)];  // fires actually written to the ring
+    if (S.raster && !(S.ctl[3] & 1u)) S.totals[3] += F;
+    S.ctl[0] = T + 1;
+}
+
+constexpr uint32_t kMaxGroups = 2 * 127 + 1;  // group table held in LDS: 2 * lmax + 1 with lmax <= 127
+
+// The tick's delivery-order groups and their first segments, by one wave, into grp[0..ng) (LDS or
+// global memory); returns the segment count (0 after an error or past the segment capacity).
+template <typename G>
+__device__ uint32_t plan_groups(const Step& S, uint32_t T, G* grp, bool report) {
     const uint32_t ng = 2 * S.lmax + 1;
-    __shared__ uint32_t wsum[1024 / kWave];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     uint32_t base = 0;
-    for (uint32_t off = 0; off < ng; off += blockDim.x) {
-        const uint32_t gi = off + threadIdx.x;
+    for (uint32_t off = 0; off < ng; off += kWave) {
+        const uint32_t gi = off + lane;
         Group g;
         g.nseg = 0;
         if (gi < ng) group_info(S, T, gi, g);
-        // block exclusive scan of nseg (1024 threads)
-        const int lane = threadIdx.x & (kWave - 1);
-        const int wid = threadIdx.x / kWave;
         const uint32_t incl = wave_incl_scan(g.nseg);
-        if (lane == kWave - 1) wsum[wid] = incl;
-        __syncthreads();
-        uint32_t wb = 0, tot = 0;
-        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) {
-            if (w < wid) wb += wsum[w];
-            tot += wsum[w];
-        }
-        __syncthreads();
         if (gi < ng) {
-            g.first = base + wb + incl - g.nseg;
-            S.groups[gi] = g;
+            g.first = base + incl - g.nseg;
+            grp[gi] = g;
         }
-        base += tot;
+        base += __shfl(incl, kWave - 1, kWave);
     }
+    if (base > S.seg_cap) {
+        if (report && lane == 0) atomicOr(&S.ctl[1], (uint32_t)kErrSegments);
+        base = 0;
+    }
+    if (S.ctl[1]) base = 0;  // after any error the run is invalid: stop doing work
+    return base;
+}
+
+// The plan of a tick whose group table does not fit LDS (lmax > 127): one wave writes the groups
+// to global memory (k_segments<true> reads them there). With fused_advance it first finishes the
+// previous tick's counters.
+__global__ void __launch_bounds__(kWave) k_plan(Step S) {
+    const uint32_t T = tick_of(S);
+    if (S.fused_advance && threadIdx.x == 0 && T > S.inj_t0) advance_tick(S, T - 1);
+    const uint32_t base = plan_groups(S, T, S.groups, true);
     if (threadIdx.x == 0) {
-        if (base > S.seg_cap) {
-            atomicOr(&S.ctl[1], (uint32_t)kErrSegments);
-            base = 0;
-        }
-        if (S.ctl[1]) base = 0;  // after any error the run is invalid: stop doing work
         S.seg_count[0] = base;
         S.seg_count[1] = 0;
     }
 }
 
+// First kernel of a tick (kScanBlocks blocks). Every block builds the tick's group table in LDS
+// (one wave; GLOBAL: k_plan built it in global memory), then takes a contiguous range of the
+// segments (one (fire, lag group) or inject run each), writes their descriptors and the range's
+// event count for the scan. With fused_advance, block 0 first finishes the previous tick's counters.
+template <bool GLOBAL>
 __global__ void __launch_bounds__(kBlock) k_segments(Step S) {
-    const uint32_t nseg = S.seg_count[0];
+    const uint32_t T = tick_of(S);
     const uint32_t ng = 2 * S.lmax + 1;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nseg; q += gridDim.x * blockDim.x) {
-        // last group whose first segment is <= q (groups are in order, empty ones share `first`)
-        uint32_t lo = 0, hi = ng;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (S.groups[mid].first <= q) lo = mid + 1; else hi = mid;
+    __shared__ Group lgrp[GLOBAL ? 1 : kMaxGroups];
+    __shared__ uint32_t nseg_s;
+    const Group* grp = GLOBAL ? S.groups : lgrp;
+    if constexpr (!GLOBAL) {
+        if (threadIdx.x < kWave) {
+            if (S.fused_advance && blockIdx.x == 0 && threadIdx.x == 0 && T > S.inj_t0) advance_tick(S, T - 1);
+            const uint32_t base = plan_groups(S, T, lgrp, blockIdx.x == 0);
+            if (threadIdx.x == 0) {
+                nseg_s = base;
+                if (blockIdx.x == 0) {
+                    S.seg_count[0] = base;
+                    S.seg_count[1] = 0;
+                }
+            }
         }
-        const Group g = S.groups[lo - 1];
+    } else {
+        if (threadIdx.x == 0) nseg_s = S.seg_count[0];
+    }
+    __syncthreads();
+    const uint32_t nseg = nseg_s;
+    // the range of the scan's block blockIdx.x (k_scan_down splits the segments the same way)
+    const uint32_t chunk = (nseg + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = min(nseg, blockIdx.x * chunk);
+    const uint32_t hi = min(nseg, lo + chunk);
+    uint32_t sum = 0;
+    for (uint32_t q = lo + threadIdx.x; q < hi; q += blockDim.x) {
+        // last group whose first segment is <= q (groups are in order, empty ones share `first`)
+        uint32_t a = 0, b = ng;
+        while (a < b) {
+            const uint32_t mid = (a + b) >> 1;
+            if (grp[mid].first <= q) a = mid + 1; else b = mid;
+        }
+        const Group g = grp[a - 1];
         uint64_t v0;
         uint32_t cnt, mult;
         segment_info(S, g, q - g.first, v0, cnt, mult);
         S.seg_v0[q] = v0;
         S.seg_n[q] = cnt;
         S.seg_mult[q] = mult;
+        sum += cnt;
     }
+    uint32_t tot;
+    block_excl_scan(sum, &tot);
+    if (threadIdx.x == 0) S.partials[blockIdx.x] = tot;
 }
 
 // Event ranges: workgroup w owns events [D*w/nwg, D*(w+1)/nwg) (a contiguous range of g), caches the
@@ -352,6 +408,46 @@ __global__ void __launch_bounds__(kPBlock) k_pcount(Step S) {
     for (uint32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) S.hist[(uint64_t)p * S.nwg + blockIdx.x] = h[p];
 }
 
+// Per partition (one block each): the scatter workgroups' event counts -> their offsets within the
+// partition's bucket, in place, and the partition's total.
+__global__ void __launch_bounds__(kBlock) k_hscan(Step S) {
+    const uint32_t p = blockIdx.x;
+    uint32_t* h = S.hist + (uint64_t)p * S.nwg;
+    const uint32_t per = (S.nwg + kBlock - 1) / kBlock;  // <= 4 (nwg <= 1024)
+    uint32_t v[4], sum = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t w = threadIdx.x * per + q;
+        v[q] = (q < per && w < S.nwg) ? h[w] : 0u;
+        sum += v[q];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(sum, &tot);
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t w = threadIdx.x * per + q;
+        if (q < per && w < S.nwg) h[w] = ex;
+        ex += v[q];
+    }
+    if (threadIdx.x == 0) S.ptotal[p] = tot;
+}
+
+// Exclusive prefix of the partition totals up to partition p (the start of bucket p), summed by the
+// whole block.
+__device__ __forceinline__ uint32_t bucket_start(const Step& S, uint32_t p) {
+    __shared__ uint32_t red[kPBlock / kWave];
+    uint32_t s = 0;
+    for (uint32_t q = threadIdx.x; q < p; q += blockDim.x) s += S.ptotal[q];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_down(s, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < blockDim.x / kWave; ++w) t += red[w];
+    __syncthreads();
+    return t;
+}
+
 // The block's events in rounds of EPT * blockDim.x consecutive positions, EPT per thread: every
 // thread runs the same number of rounds (f gets ok = false past the end), so f may contain block
 // barriers.
@@ -433,12 +529,23 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
     uint32_t* cnt = dyn + np;          // records taken since the last flush of partition p
     E* buf = (E*)(dyn + 2 * np);       // [np][ch] events
     uint16_t* buft = (uint16_t*)(buf + (size_t)np * ch);  // [np][ch] target indices
-    for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
-        base[p] = S.hist[(uint64_t)p * S.nwg + blockIdx.x];
-        cnt[p] = 0;
+    {
+        // bucket starts: exclusive scan of the partition totals, one per thread per block scan
+        uint32_t run = 0;
+        for (uint32_t p0 = 0; p0 < np; p0 += blockDim.x) {
+            const uint32_t p = p0 + threadIdx.x;
+            const uint32_t v = p < np ? S.ptotal[p] : 0u;
+            uint32_t tot;
+            const uint32_t ex = run + block_excl_scan_n<kPBlock>(v, &tot);
+            if (p < np) {
+                base[p] = ex + S.hist[(uint64_t)p * S.nwg + blockIdx.x];
+                cnt[p] = 0;
+            }
+            run += tot;
+        }
     }
     __syncthreads();
-    const uint32_t T = S.ctl[0];
+    const uint32_t T = tick_of(S);
     const uint32_t shift = S.pshift;
     const uint32_t mask = (1u << shift) - 1;
     E* out = (E*)S.prec;
@@ -741,10 +848,9 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
     // partition holds 1,024 to 4,096 targets)
     constexpr uint32_t kScanPer = 1024 / BLOCK;
     const uint32_t p = blockIdx.x;
-    const uint32_t T = S.ctl[0];
-    const uint32_t start = S.hist[(uint64_t)p * S.nwg];
-    const uint32_t end = p + 1 < S.nparts ? S.hist[(uint64_t)(p + 1) * S.nwg] : S.seg_count[1];
-    const uint32_t c = end - start;
+    const uint32_t T = tick_of(S);
+    const uint32_t start = bucket_start(S, p);
+    const uint32_t c = S.ptotal[p];
     const E* src = (const E*)S.prec + start;
     const uint16_t* tls = S.ptl + start;
     constexpr uint32_t kNoTarget = 0xFFFFu;
@@ -755,8 +861,8 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
             const uint32_t nwg = S.nwg;
             bool big = false;
             for (uint32_t w = threadIdx.x; w < nwg; w += BLOCK) {
-                aux[w] = S.hist[(uint64_t)p * nwg + w] - start;
-                const uint32_t nxt = w + 1 < nwg ? S.hist[(uint64_t)p * nwg + w + 1] - start : c;
+                aux[w] = S.hist[(uint64_t)p * nwg + w];
+                const uint32_t nxt = w + 1 < nwg ? S.hist[(uint64_t)p * nwg + w + 1] : c;
                 big |= nxt - aux[w] > kCap;
             }
             if (threadIdx.x == 0) aux[nwg] = c;
@@ -834,11 +940,20 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
     fold_rounds<BLOCK, WIDE, STDP, CODE_HERE, CODE2>(S, lds_stage, aux, off, c, start, T);
 }
 
-// rank-ordered fire list of tick T into the fire ring (and the raster)
+// Rank-ordered fire list of tick T into the fire ring (and the raster). kScanBlocks blocks, each over
+// a contiguous range of the fire bitmap's words, as k_scan_reduce summed them: a block's first
+// rank is the sum of the partials before it (the scan's down-sweep, done here), F their total.
 __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
-    const uint32_t T = S.ctl[0];
-    const uint32_t F = S.ctl[2];
+    const uint32_t T = tick_of(S);
     const uint32_t nwords = (S.seg_count[1] + 31u) >> 5;
+    uint32_t before = 0, all = 0;
+    for (uint32_t j = threadIdx.x; j < gridDim.x; j += blockDim.x) {
+        const uint32_t v = S.partials[j];
+        all += v;
+        before += j < blockIdx.x ? v : 0u;
+    }
+    block_sum2(before, all);
+    const uint32_t F = all;
     const uint64_t head = S.totals[2];
     const uint32_t R = S.lmax + 1;
     // entries of ticks older than T+1-lmax may be overwritten (older than T-lmax with lazy code
@@ -849,11 +964,18 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
     const uint64_t rfill = S.totals[3];
     // the raster is a recording aid: when it is full, recording stops and the run goes on
     const bool rfits = S.raster && !(S.ctl[3] & 1u) && rfill + F <= S.raster_cap;
-    for (uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x; wi < nwords; wi += gridDim.x * blockDim.x) {
-        uint32_t bits = S.fire_bits[wi];
+    const uint32_t chunk = (nwords + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = min(nwords, blockIdx.x * chunk);
+    const uint32_t hi = min(nwords, lo + chunk);
+    uint32_t base = before;
+    for (uint32_t off = lo; off < hi; off += blockDim.x) {
+        const uint32_t wi = off + threadIdx.x;
+        uint32_t bits = wi < hi ? S.fire_bits[wi] : 0u;
+        uint32_t tot;
+        uint32_t pos = base + block_excl_scan(__popc(bits), &tot);
+        base += tot;
         if (!bits) continue;
         S.fire_bits[wi] = 0;
-        uint32_t pos = S.fire_word_off[wi];
         while (bits) {
             const uint32_t b = __ffs(bits) - 1;
             bits &= bits - 1;
@@ -886,6 +1008,7 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
         if (err) atomicOr(&S.ctl[1], err);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S.ctl[2] = F;
         if (!fits) atomicOr(&S.ctl[1], (uint32_t)kErrFireLog);
         if (S.raster && !rfits) atomicOr(&S.ctl[3], 1u);
         S.tick_fstart[T % R] = head;
@@ -895,7 +1018,7 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
 
 // post-synaptic scan: one wave per (first) fire of tick T
 __global__ void __launch_bounds__(kBlock) k_post(Step S) {
-    const uint32_t T = S.ctl[0];
+    const uint32_t T = tick_of(S);
     const uint32_t R = S.lmax + 1;
     const uint32_t F = S.tick_fcount[T % R];
     const uint64_t start = S.tick_fstart[T % R];
@@ -912,18 +1035,9 @@ __global__ void __launch_bounds__(kBlock) k_post(Step S) {
     }
 }
 
-__global__ void k_advance(Step S, uint32_t t0) {
-    const uint32_t T = S.ctl[0];
-    const uint32_t F = S.ctl[2];
-    const uint32_t D = S.seg_count[1];
-    S.tick_delivered[T - t0] = D;
-    S.tick_fired[T - t0] = F;
-    S.totals[0] += D;
-    S.totals[1] += F;
-    S.totals[2] += S.tick_fcount[T % (S.lmax + 1)];  // fires actually written to the ring
-    if (S.raster && !(S.ctl[3] & 1u)) S.totals[3] += F;
-    S.ctl[0] = T + 1;
-}
+// the counters of tick T (every tick of a HIP graph replay; the last tick of a run of stream
+// launches, whose earlier ticks are finished by the next tick's k_segments)
+__global__ void k_advance(Step S) { advance_tick(S, tick_of(S)); }
 
 // the STDP records' update counters, record order
 __global__ void k_stdp_counts(const StdpRec* stdp, uint64_t nt, uint32_t* upd, uint32_t* ltp) {
@@ -1370,6 +1484,10 @@ struct snn_net {
     uint32_t* slot_of = nullptr;
     uint32_t* tpos_conn_dev = nullptr;
     uint64_t n_tpos = 0;
+    // the next tick to launch, and whether the last launched tick still needs its counters
+    // finished (stream launches leave them to the next tick's first kernel)
+    uint64_t tick_next = 0;
+    bool advance_pending = false;
     // profiling
     bool profiling = false;
     uint32_t prof_mask = ~0u;           // kernels timed (bit = KernelId)
@@ -1423,9 +1541,8 @@ void launch_scan(snn_net* net, Src src, uint32_t* out, uint32_t* total, uint64_t
 constexpr uint32_t kScatterDynLds = 64 * 1024;
 
 template <bool WIDE, bool CODE_HERE, bool CODE2>
-void launch_scatter(snn_net* net) {
+void launch_scatter(snn_net* net, Step& S) {
     typedef typename EntryOf<WIDE, CODE_HERE>::type E;
-    Step& S = net->S;
     // up to 256 bytes of events per flush, as many as the LDS holds (26 per partition at 244
     // partitions): longer flushed runs leave fewer partly written cache lines
     const uint32_t want = WIDE ? 16u : 32u;
@@ -1434,20 +1551,21 @@ void launch_scatter(snn_net* net) {
     uint32_t ch = (uint32_t)std::min<uint64_t>(want, room / ((uint64_t)S.nparts * kRecBytes));
     if (ch < 4) ch = 0;
     S.chunk = ch;
+    net->S.chunk = ch;
     const size_t lds = 8ull * S.nparts + (size_t)S.nparts * ch * kRecBytes;
     hipLaunchKernelGGL((k_pscatter<WIDE, CODE_HERE, CODE2>), dim3(S.nwg), dim3(kPBlock), lds, net->stream, S);
 }
 
 template <bool WIDE, bool STDP, bool CODE_HERE, bool CODE2>
-void launch_fold(snn_net* net) {
+void launch_fold(snn_net* net, const Step& S) {
     // per-target counters: one array for the 256-thread fold, two for the 1024-thread fold's chunks
-    const size_t lds = ((1u << net->S.pshift) + 1) * 4;
+    const size_t lds = ((1u << S.pshift) + 1) * 4;
     if (net->fold_threads == (uint32_t)kSmallFold)
-        hipLaunchKernelGGL((k_pfold<kSmallFold, WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kSmallFold),
-                           lds, net->stream, net->S);
+        hipLaunchKernelGGL((k_pfold<kSmallFold, WIDE, STDP, CODE_HERE, CODE2>), dim3(S.nparts), dim3(kSmallFold),
+                           lds, net->stream, S);
     else
-        hipLaunchKernelGGL((k_pfold<kPBlock, WIDE, STDP, CODE_HERE, CODE2>), dim3(net->S.nparts), dim3(kPBlock), 2 * lds,
-                           net->stream, net->S);
+        hipLaunchKernelGGL((k_pfold<kPBlock, WIDE, STDP, CODE_HERE, CODE2>), dim3(S.nparts), dim3(kPBlock), 2 * lds,
+                           net->stream, S);
 }
 
 // The per-slot compact code records: the network's own, or with per-source code patterns a
@@ -1484,26 +1602,26 @@ void allow_scatter_lds() {
     });
 }
 
-void dispatch_scatter(snn_net* net) {
+void dispatch_scatter(snn_net* net, Step& S) {
     // (wide, code_here, code2); code_here needs one code per synapse
     if (!net->wide) {
-        if (net->code2) launch_scatter<false, true, true>(net); else launch_scatter<false, true, false>(net);
+        if (net->code2) launch_scatter<false, true, true>(net, S); else launch_scatter<false, true, false>(net, S);
     } else if (net->code_here) {
-        if (net->code2) launch_scatter<true, true, true>(net); else launch_scatter<true, true, false>(net);
+        if (net->code2) launch_scatter<true, true, true>(net, S); else launch_scatter<true, true, false>(net, S);
     } else {
-        if (net->code2) launch_scatter<true, false, true>(net); else launch_scatter<true, false, false>(net);
+        if (net->code2) launch_scatter<true, false, true>(net, S); else launch_scatter<true, false, false>(net, S);
     }
 }
 
-void dispatch_fold(snn_net* net) {
+void dispatch_fold(snn_net* net, const Step& S) {
     if (!net->wide) {
-        if (net->code2) launch_fold<false, false, true, true>(net); else launch_fold<false, false, true, false>(net);
+        if (net->code2) launch_fold<false, false, true, true>(net, S); else launch_fold<false, false, true, false>(net, S);
         return;
     }
 #define SNN_FOLD(ST, CH)                                                         \
     if (net->stdp == ST && net->code_here == CH) {                             \
-        if (net->code2) launch_fold<true, ST, CH, true>(net);                  \
-        else launch_fold<true, ST, CH, false>(net);                            \
+        if (net->code2) launch_fold<true, ST, CH, true>(net, S);                  \
+        else launch_fold<true, ST, CH, false>(net, S);                            \
         return;                                                                \
     }
     SNN_FOLD(true, true)
@@ -1558,45 +1676,82 @@ struct Timer {
     }
 };
 
-void launch_tick(snn_net* net, uint32_t t0, Timer& tm) {
-    Step& S = net->S;
+// One tick's launches. Stream launches carry the tick as an argument and leave the tick's counters
+// to the next tick's k_segments (or to flush_advance); a HIP graph capture (graph) reads the tick
+// from device memory and finishes each tick with k_advance, so every replay steps on.
+void launch_tick(snn_net* net, Timer& tm, bool graph = false) {
+    Step S = net->S;
     hipStream_t st = net->stream;
-    tm.begin(KPlan);
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, S);
-    tm.end();
-    tm.begin(KSegments);
-    hipLaunchKernelGGL(k_segments, dim3(net->grid), dim3(kBlock), 0, st, S);
-    tm.end();
+    if (graph) {
+        S.tick = kNone;
+        S.fused_advance = 0;
+    } else {
+        S.tick = (uint32_t)net->tick_next;
+        S.fused_advance = net->advance_pending ? 1u : 0u;
+    }
+    if (2 * S.lmax + 1 > kMaxGroups) {
+        // the group table does not fit LDS: planned by one wave in global memory
+        tm.begin(KPlan);
+        hipLaunchKernelGGL(k_plan, dim3(1), dim3(kWave), 0, st, S);
+        tm.end();
+        S.fused_advance = 0;
+        tm.begin(KSegments);
+        hipLaunchKernelGGL(k_segments<true>, dim3(kScanBlocks), dim3(kBlock), 0, st, S);
+        tm.end();
+    } else {
+        tm.begin(KSegments);
+        hipLaunchKernelGGL(k_segments<false>, dim3(kScanBlocks), dim3(kBlock), 0, st, S);
+        tm.end();
+    }
     tm.begin(KScanSeg);
-    launch_scan(net, PlainSrc{S.seg_n, &S.seg_count[0], 0}, S.seg_off, &S.seg_count[1],
-                std::min<uint64_t>(S.ev_cap, kMaxEventsPerTick), kErrEvents, &S.seg_count[0]);
+    hipLaunchKernelGGL(k_scan_down<PlainSrc>, dim3(kScanBlocks), dim3(kBlock), 0, st,
+                       PlainSrc{S.seg_n, &S.seg_count[0], 0}, (const uint32_t*)S.partials, S.seg_off, &S.seg_count[1],
+                       std::min<uint64_t>(S.ev_cap, kMaxEventsPerTick), (uint32_t)kErrEvents, &S.ctl[1], &S.seg_count[0]);
     tm.end();
     tm.begin(KCount);
     hipLaunchKernelGGL(k_pcount, dim3(S.nwg), dim3(kPBlock), S.nparts * 4, st, S);
     tm.end();
     tm.begin(KScanTgt);
-    launch_scan(net, PlainSrc{S.hist, nullptr, S.nparts * S.nwg}, S.hist, nullptr, 0, 0, nullptr);
+    hipLaunchKernelGGL(k_hscan, dim3(S.nparts), dim3(kBlock), 0, st, S);
     tm.end();
     tm.begin(KScatter);
-    dispatch_scatter(net);
+    dispatch_scatter(net, S);
     tm.end();
     tm.begin(KFold);
-    dispatch_fold(net);
+    dispatch_fold(net, S);
     tm.end();
     tm.begin(KScanFire);
-    launch_scan(net, PopcSrc{S.fire_bits, &S.seg_count[1]}, S.fire_word_off, &S.ctl[2], 0, 0, nullptr);
+    hipLaunchKernelGGL(k_scan_reduce<PopcSrc>, dim3(kScanBlocks), dim3(kBlock), 0, st, PopcSrc{S.fire_bits, &S.seg_count[1]},
+                       S.partials);
     tm.end();
     tm.begin(KFireWrite);
-    hipLaunchKernelGGL(k_fire_write, dim3(net->grid), dim3(kBlock), 0, st, S);
+    hipLaunchKernelGGL(k_fire_write, dim3(kScanBlocks), dim3(kBlock), 0, st, S);
     tm.end();
     if (net->stdp) {
         tm.begin(KPost);
         hipLaunchKernelGGL(k_post, dim3(net->grid), dim3(kBlock), 0, st, S);
         tm.end();
     }
+    if (graph) {
+        tm.begin(KAdvance);
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, S);
+        tm.end();
+    } else {
+        net->advance_pending = true;
+        net->tick_next += 1;
+    }
+}
+
+// The counters of the last tick launched by stream launches, when the next tick's k_segments will
+// not finish them (end of a run, before graph replays, before a trial is read back).
+void flush_advance(snn_net* net, Timer& tm) {
+    if (!net->advance_pending) return;
+    Step S = net->S;
+    S.tick = (uint32_t)(net->tick_next - 1);
     tm.begin(KAdvance);
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, S, t0);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, net->stream, S);
     tm.end();
+    net->advance_pending = false;
 }
 
 int fail(int code, const std::string& msg) {
@@ -1668,7 +1823,7 @@ void ensure_graph(snn_net* net, uint32_t t0, TickGraph& g) {
     Timer off{net};
     off.enabled = false;
     check(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    for (uint32_t i = 0; i < sc.graph; ++i) launch_tick(net, t0, off);
+    for (uint32_t i = 0; i < sc.graph; ++i) launch_tick(net, off, true);
     check(hipStreamEndCapture(net->stream, &g.graph), "hipStreamEndCapture");
     check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     g.sc = sc;
@@ -1679,10 +1834,15 @@ void run_ticks(snn_net* net, uint32_t t0, uint64_t k, Timer& tm, TickGraph& g) {
     uint64_t i = 0;
     const uint32_t G = net->profiling ? 0 : net->graph_ticks;
     if (G && k >= G) {
+        flush_advance(net, tm);
         ensure_graph(net, t0, g);
-        for (; i + G <= k; i += G) check(hipGraphLaunch(g.exec, net->stream), "hipGraphLaunch");
+        for (; i + G <= k; i += G) {
+            check(hipGraphLaunch(g.exec, net->stream), "hipGraphLaunch");
+            net->tick_next += G;
+        }
     }
-    for (; i < k; ++i) launch_tick(net, t0, tm);
+    for (; i < k; ++i) launch_tick(net, tm);
+    flush_advance(net, tm);
 }
 
 // Online autotuning on the run's own ticks. Coordinate search, one schedule field at a time (graph
@@ -1906,9 +2066,10 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     net->ncu = (uint32_t)std::max(1, ncu);
     S.nwg = 2 * net->ncu;
     if (const char* e = std::getenv("SNNCUDA_SCATTER_WGS")) S.nwg = (uint32_t)std::max(1, std::atoi(e));
+    S.nwg = std::min<uint32_t>(S.nwg, kMaxScatterWgs);
     // the histogram holds the largest schedule the autotuner may pick: 1024-neuron partitions and
     // four workgroups per CU
-    net->max_nwg = std::max<uint32_t>(S.nwg, 4 * net->ncu);
+    net->max_nwg = std::min<uint32_t>(kMaxScatterWgs, std::max<uint32_t>(S.nwg, 4 * net->ncu));
     net->hist_cap_parts = (uint32_t)(((uint64_t)n + 1023) >> 10);
     S.hist = dev_alloc<uint32_t>(own, (uint64_t)net->hist_cap_parts * net->max_nwg);
     S.prec = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
@@ -1917,12 +2078,15 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     S.fire_bits = dev_alloc<uint32_t>(own, ev_cap / 32);
     S.fire_neuron = dev_alloc<uint32_t>(own, ev_cap, false);
     S.fire_mult = dev_alloc<uint32_t>(own, ev_cap, false);
-    S.fire_word_off = dev_alloc<uint32_t>(own, ev_cap / 32, false);
     S.ctl = dev_alloc<uint32_t>(own, 4);
     S.totals = dev_alloc<uint64_t>(own, 4);
     S.raster_cap = o.raster_capacity ? o.raster_capacity : (1ull << 22);
     S.raster = dev_alloc<uint32_t>(own, S.raster_cap, false);
     if (!net->partials) net->partials = dev_alloc<uint32_t>(own, kScanBlocks);
+    S.partials = net->partials;
+    S.ptotal = dev_alloc<uint32_t>(own, net->hist_cap_parts);
+    S.tick = kNone;
+    S.fused_advance = 0;
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) cus = prop.multiProcessorCount;
@@ -2584,17 +2748,21 @@ int run_until_impl(snn_net* net, uint64_t inclusive_tick, bool static_staging) {
             check(hipMemcpy(S.ctl, &tick32, 4, hipMemcpyHostToDevice), "set tick");
         }
         Timer tm{net};
+        net->tick_next = t0;
+        net->advance_pending = false;
         if (static_staging) {
             tm.enabled = false;
-            for (uint64_t t = 0; t < nt; ++t) launch_tick(net, (uint32_t)t0, tm);
+            for (uint64_t t = 0; t < nt; ++t) launch_tick(net, tm);
+            flush_advance(net, tm);
         } else if (net->profiling) {
             for (uint64_t t = 0; t < nt; ++t) {
-                launch_tick(net, (uint32_t)t0, tm);
+                launch_tick(net, tm);
                 if (tm.marks.size() > 4096) {
                     check(hipStreamSynchronize(net->stream), "sync");
                     tm.collect();
                 }
             }
+            flush_advance(net, tm);
         } else {
             TickGraph g;
             const uint64_t done = net->tune_trial ? autotune(net, (uint32_t)t0, nt, tm, g) : 0;
