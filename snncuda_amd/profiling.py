@@ -10,6 +10,8 @@
       each kernel (ramp-up ticks). Writes a short text summary.
   python -m snncuda_amd.profiling calibrate --fetch F.db --write W.db --mib N --out FILE
       Ratio of the copy kernel's counter bytes to the bytes it moves.
+  python -m snncuda_amd.profiling counters DB... [--skip K] [--kernels PREFIX...] --cmd "..." --out FILE
+      Mean per dispatch of every counter in --pmc databases, per kernel.
 """
 from __future__ import annotations
 
@@ -115,6 +117,30 @@ def cmd_summary(a):
     print(text)
 
 
+def cmd_counters(a):
+    """Mean per dispatch of every counter in the given --pmc databases, per kernel (first --skip
+    dispatches of each kernel left out), for the kernels named by --kernels prefixes."""
+    per = {}
+    for db in a.dbs:
+        c = sqlite3.connect(db).cursor()
+        names = [r[0] for r in c.execute("select distinct counter_name from pmc_events")]
+        for cn in names:
+            for k, v in counter_values(db, cn).items():
+                if a.kernels and not any(k.startswith(p) for p in a.kernels):
+                    continue
+                v = v[a.skip:] if len(v) > a.skip else v
+                per.setdefault(k, {})[cn] = statistics.mean(v)
+    lines = [f"command: {a.cmd}", f"source: rocprofv3 --pmc (values per dispatch as rocprofv3 reports them), "
+             f"mean over dispatches after the first {a.skip} of each kernel", ""]
+    for k in sorted(per):
+        lines.append(k)
+        for cn in sorted(per[k]):
+            lines.append(f"   {cn:28s} {per[k][cn]:16.1f}")
+    text = "\n".join(lines) + "\n"
+    open(a.out, "w").write(text)
+    print(text)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd_name", required=True)
@@ -139,8 +165,14 @@ def main(argv=None):
     s.add_argument("--out", required=True)
     s.add_argument("--json")
     s.add_argument("--calibration")
+    n = sub.add_parser("counters")
+    n.add_argument("dbs", nargs="+")
+    n.add_argument("--skip", type=int, default=0)
+    n.add_argument("--kernels", nargs="*")
+    n.add_argument("--cmd", required=True)
+    n.add_argument("--out", required=True)
     a = ap.parse_args(argv)
-    {"copy": cmd_copy, "calibrate": cmd_calibrate, "summary": cmd_summary}[a.cmd_name](a)
+    {"copy": cmd_copy, "calibrate": cmd_calibrate, "summary": cmd_summary, "counters": cmd_counters}[a.cmd_name](a)
 
 
 if __name__ == "__main__":
