@@ -108,6 +108,8 @@ class Results:
     synapse_patterns: List[tuple]
     delivered: int
     fired: int
+    neuron_words: np.ndarray = None   # the encoded pattern blobs as written (see ref_harness.cpp Blob)
+    synapse_words: np.ndarray = None
 
 
 def _decode_blob(words: np.ndarray, count: int) -> List[tuple]:
@@ -158,7 +160,7 @@ def read_results(path, patterns: bool = True) -> Results:
     delivered, fired = struct.unpack_from("<2Q", data, off)
     return Results(dpt, fpt, raster, neuron, synapse,
                    _decode_blob(nwords, n) if patterns else [],
-                   _decode_blob(swords, e) if patterns else [], delivered, fired)
+                   _decode_blob(swords, e) if patterns else [], delivered, fired, nwords, swords)
 
 
 def run_oracle_file(net_path, out_path) -> float:

@@ -22,6 +22,7 @@ from snncuda_amd import workloads as W  # noqa: E402
 from tests import kat_nets as K  # noqa: E402
 
 HERE = pathlib.Path(__file__).resolve().parent
+SYNAPSE_WORDS_KEPT = 100_000
 
 
 def cases():
@@ -54,7 +55,13 @@ def save(name, net, res: "oracle.Results") -> None:
         t_window_start=n["t_window_start"], t_n_offsets=n["t_n_offsets"],
         weight=s["weight"], pre_count=s["pre_count"], upd_count=s["upd_count"], last_pre=s["last_pre"],
         last_post=s["last_post"], c_match=s["c_match"], c_learned=s["c_learned"],
-        delivered=np.array(res.delivered), fired=np.array(res.fired))
+        delivered=np.array(res.delivered), fired=np.array(res.fired),
+        # learned-pattern contents: every neuron's candidate offsets and learned patterns, and every
+        # synapse's code offsets and learned code patterns, in the harness's word encoding (the
+        # synapse words of large cases as a SHA-256, to keep the fixtures small)
+        neuron_words=res.neuron_words,
+        **({"synapse_words": res.synapse_words} if res.synapse_words.size <= SYNAPSE_WORDS_KEPT else
+           {"synapse_words_sha256": np.array(hashlib.sha256(res.synapse_words.tobytes()).hexdigest())}))
 
 
 def main(names=None):
@@ -63,7 +70,7 @@ def main(names=None):
         if names and name not in names:
             continue
         net = fn()
-        res = oracle.run_reference(net, patterns=False)
+        res = oracle.run_reference(net, patterns=True)
         save(name, net, res)
         print(f"{name}: delivered={res.delivered} fired={res.fired}")
 

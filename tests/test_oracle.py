@@ -27,6 +27,26 @@ def test_oracle_matches_reference_fixture(name):
     assert not bad, f"{name}: oracle differs from the reference in {bad}"
 
 
+@pytest.mark.parametrize("name", sorted(G.CASES))
+def test_oracle_learned_pattern_contents_match_reference_fixture(name):
+    # Every neuron's candidate offsets and learned patterns and every synapse's code offsets and
+    # learned code patterns, word for word as the reference runtime left them (the fixture holds the
+    # synapse words of the large cases as a SHA-256).
+    import hashlib
+    net = G.CASES[name]()
+    g = G.load(name)
+    res = oracle.run_oracle(net, patterns=True)
+    assert np.array_equal(res.neuron_words, g["neuron_words"]), f"{name}: neuron pattern contents differ"
+    if "synapse_words" in g:
+        assert np.array_equal(res.synapse_words, g["synapse_words"]), f"{name}: synapse code patterns differ"
+    else:
+        assert hashlib.sha256(res.synapse_words.tobytes()).hexdigest() == str(g["synapse_words_sha256"]), \
+            f"{name}: synapse code patterns differ"
+    # the decoded counts agree with the count fields
+    assert [len(p) for _, p in res.neuron_patterns] == list(res.neuron["t_learned"])
+    assert [len(o) for o, _ in res.neuron_patterns] == list(res.neuron["t_n_offsets"])
+
+
 def _run(net):
     o = oracle.Oracle(net)
     o.run_commands(net.commands)
