@@ -220,7 +220,9 @@ int snn_net_read_inject_log(snn_net* net, uint64_t* n_out, uint32_t* neuron, uin
  * snn_adjoint_forward writes the trace [ticks][n_neurons] and keeps checkpoints every 64 ticks;
  * snn_adjoint_backward takes dL/dtrace [ticks][n_neurons] for the last forward's parameters,
  * recomputes each 64-tick segment from its checkpoint and writes dL/dweight (device, n_synapses) and
- * dL/dparams (host, 8 doubles). */
+ * dL/dparams (host, 8 doubles). The backward reads the weights through the last forward's d_weight,
+ * so that buffer must stay valid and unchanged until the next snn_adjoint_forward or
+ * snn_adjoint_destroy. */
 typedef struct snn_adjoint snn_adjoint;
 int snn_adjoint_create(snn_net* run, const snn_net_desc* desc, uint64_t ticks, snn_adjoint** out);
 int snn_adjoint_destroy(snn_adjoint* adj);

@@ -42,6 +42,7 @@ class Adjoint:
         if getattr(self, "_h", None):
             lib().snn_adjoint_destroy(self._h)
             self._h = None
+        self._weight = None
 
     def __del__(self):
         self.close()
@@ -62,6 +63,8 @@ class Adjoint:
         torch.cuda.synchronize(weight.device)
         _check(lib().snn_adjoint_forward(self._h, ctypes.c_void_p(weight.data_ptr()), _p(pr),
                                          ctypes.c_void_p(trace.data_ptr())))
+        # backward reads the weights through this tensor's memory: hold it until the next forward
+        self._weight = weight
         self.token += 1
 
     def backward(self, grad_trace, grad_weight) -> np.ndarray:

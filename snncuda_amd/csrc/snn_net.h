@@ -106,6 +106,26 @@ struct HostNet {
     std::vector<NeuronParams> params;
 };
 
+// Target partitions on one device. The scatter keeps two counters per partition in 64 KB of dynamic
+// LDS (8 B each) and the count kernel one (4 B each), so at most 8,192 partitions fit: 33,554,432
+// neurons in 4096-neuron partitions.
+constexpr uint32_t kMaxPartitions = 8192;
+
+inline uint64_t partitions_of(uint64_t n, uint32_t pshift) { return (n + (1ull << pshift) - 1) >> pshift; }
+
+// The default partition size for n neurons (2^10, 2^11 from 1,025 neurons, 2^12 from 512k), and the
+// error a network too large for one device raises when it is created.
+inline uint32_t default_partition_shift(uint64_t n) {
+    const uint32_t pshift = n <= 1024 ? 10 : (n < (1u << 19) ? 11 : 12);
+    if (partitions_of(n, 12) > kMaxPartitions)
+        throw std::invalid_argument("network of " + std::to_string(n) + " neurons needs " +
+                                    std::to_string(partitions_of(n, 12)) +
+                                    " target partitions of 4096 neurons; one device holds at most " +
+                                    std::to_string(kMaxPartitions) + " (" +
+                                    std::to_string((uint64_t)kMaxPartitions << 12) + " neurons)");
+    return partitions_of(n, pshift) > kMaxPartitions ? 12u : pshift;
+}
+
 inline uint32_t lag_of(uint32_t delay, uint32_t off) {
     const uint64_t l = (uint64_t)delay + off;
     return l == 0 ? 1u : (uint32_t)std::min<uint64_t>(l, 0xFFFFFFFFull);

@@ -1789,7 +1789,7 @@ void apply_schedule(snn_net* net, const Schedule& sc) {
     Step& S = net->S;
     if (sc.pshift < 10 || sc.pshift > 12) throw std::invalid_argument("partition_shift must be in [10, 12]");
     const uint64_t parts = ((uint64_t)S.n + (1ull << sc.pshift) - 1) >> sc.pshift;
-    if (parts > net->hist_cap_parts || parts * 4 > 65536) throw std::invalid_argument("partition_shift too small");
+    if (parts > net->hist_cap_parts || parts > kMaxPartitions) throw std::invalid_argument("partition_shift too small");
     if (sc.nwg < 1 || sc.nwg > net->max_nwg) throw std::invalid_argument("scatter_workgroups out of range");
     if (sc.graph > 1024) throw std::invalid_argument("graph_ticks above 1024");
     if (sc.fold != (uint32_t)kPBlock && sc.fold != (uint32_t)kSmallFold)
@@ -1879,7 +1879,7 @@ uint64_t autotune(snn_net* net, uint32_t t0, uint64_t nt, Timer& tm, TickGraph& 
         std::vector<Schedule> c;
         for (uint32_t ps = 10; ps <= 12; ++ps) {
             const uint64_t parts = ((uint64_t)net->S.n + (1ull << ps) - 1) >> ps;
-            if (parts > net->hist_cap_parts || parts * 4 > 65536) continue;
+            if (parts > net->hist_cap_parts || parts > kMaxPartitions) continue;
             if (ps > 10 && net->S.n <= (1u << (ps - 1))) continue;  // one partition already
             b.pshift = ps;
             c.push_back(b);
@@ -2054,10 +2054,9 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     // scatter's write-combining buffers hold a full chunk for each; folded in rounds of 1024
     // neurons. Networks of at most 1024 neurons use a single 1024-neuron partition. Measured on the
     // 1M-neuron workload: 4096-neuron partitions and two scatter workgroups per CU are fastest.
-    uint32_t pshift = n <= 1024 ? 10 : (n < (1u << 19) ? 11 : 12);
+    uint32_t pshift = default_partition_shift(n);
     if (const char* e = std::getenv("SNNCUDA_PART_SHIFT")) pshift = std::max(10, std::min(12, std::atoi(e)));
-    if (((uint64_t)n + (1ull << pshift) - 1) >> pshift > 65535)
-        throw std::invalid_argument("too many neurons for 4096-neuron partitions on one device");
+    if (partitions_of(n, pshift) > kMaxPartitions) pshift = 12;
     S.pshift = pshift;
     S.nparts = (uint32_t)(((uint64_t)n + (1ull << pshift) - 1) >> pshift);
     int ncu = 256;
@@ -2123,6 +2122,7 @@ std::unique_ptr<snn_net> create_net(const NetDesc& d, const snn_options& o) {
     if (d.n_synapses && (!d.source || !d.target || !d.weight || !d.max_weight || !d.delay || !d.compartment ||
                          !d.receptor || !d.plastic))
         throw std::invalid_argument("null synapse array");
+    (void)default_partition_shift(d.n_neurons);  // a network too large for one device fails here
     HostNet h = build_host_net(d);
 
     std::unique_ptr<snn_net> net(new snn_net());
@@ -2201,6 +2201,7 @@ std::unique_ptr<snn_net> generate_nets(const snn_ei_desc* g, uint64_t m, const s
         any_plastic |= N.plastic != 0;
     }
     const uint64_t n = noff[m];
+    (void)default_partition_shift(n);  // a network too large for one device fails here
     const uint32_t W = lmax + 1;
     if (W > kGenBins) throw std::invalid_argument("delay_max + code_max above 127");
     if (n * W > (1ull << 31)) throw std::invalid_argument("neurons x (max lag + 1) exceeds the group-table limit");

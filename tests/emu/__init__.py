@@ -44,6 +44,8 @@ def lib():
         L.emu_pack_check.argtypes = [ctypes.c_uint64] + [P] * 4 + [ctypes.c_uint64] + [P] * 10 + \
             [ctypes.c_uint32, ctypes.c_char_p, ctypes.c_uint64]
         L.emu_pack_check.restype = ctypes.c_int
+        L.emu_partition_shift.argtypes = [ctypes.c_uint64, ctypes.c_char_p, ctypes.c_uint64]
+        L.emu_partition_shift.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -136,3 +138,10 @@ def pack_check(net, threads):
     rc = lib().emu_pack_check(net.n_neurons, *[_p(a) for a in keep[:4]], net.n_synapses,
                               *[_p(a) for a in keep[4:]], threads, msg, 512)
     return rc, msg.value.decode()
+
+
+def partition_shift(n):
+    """(shift, error message) of the partition size snn_net_create picks for n neurons."""
+    buf = ctypes.create_string_buffer(512)
+    r = lib().emu_partition_shift(int(n), buf, 512)
+    return r, buf.value.decode()

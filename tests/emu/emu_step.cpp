@@ -473,6 +473,16 @@ void emu_synapses(void* p, float* w, uint64_t* pre, uint64_t* cmatch, uint64_t* 
 }
 
 // the per-target sort alone, on k entries of `width` 32-bit words (gk first) in place
+// The partition shift snn_net_create picks for n neurons, or -1 with the error it raises in msg.
+int emu_partition_shift(uint64_t n, char* msg, uint64_t cap) {
+    try {
+        return (int)default_partition_shift(n);
+    } catch (const std::exception& e) {
+        std::snprintf(msg, cap, "%s", e.what());
+        return -1;
+    }
+}
+
 // narrow_g: bit 0 every g below 2^27, bit 1 the small networks
 void emu_sort_by_g(uint32_t* words, uint32_t k, uint32_t narrow_g, uint32_t width) {
     const bool narrow = narrow_g & 1u, small = narrow_g & 2u;

@@ -69,3 +69,15 @@ def test_per_tick_temporal_update_equals_single_observations(sim, max_refs):
         ticks = np.cumsum(gaps).astype(np.uint32)
         times = rng.choice([1, 1, 2, 3, 7, 12, 30], size=m).astype(np.uint32)
         assert temporal_check(ticks, times, window, sim, max_refs) == m, (trial, window)
+
+
+def test_network_too_large_for_one_device_is_refused_at_create():
+    # The scatter keeps 8 B per target partition in 64 KB of LDS: at most 8,192 partitions of 4,096
+    # neurons. Past that, create raises a clear error instead of a launch failure at the first tick.
+    from tests.emu import partition_shift
+    assert partition_shift(1000)[0] == 10
+    assert partition_shift(100_000)[0] == 11
+    assert partition_shift(1_010_000)[0] == 12
+    assert partition_shift(8192 * 4096)[0] == 12
+    r, msg = partition_shift(8192 * 4096 + 1)
+    assert r == -1 and "8192" in msg and "target partitions" in msg
