@@ -101,7 +101,6 @@ struct HostNet {
     pvec<uint64_t> in_ptr;   // n+1 target-major plastic ranges
     pvec<StdpRec> stdp;      // per tpos
     pvec<uint32_t> tpos_conn;  // per tpos: connectome index
-    pvec<float> wmax;        // distinct min(2, max_weight) values (<= 256)
     // neurons
     std::vector<NeuronParams> params;
 };
@@ -449,30 +448,11 @@ inline HostNet build_host_net(const NetDesc& d, unsigned threads = 0) {
         for (unsigned t = 0; t < T; ++t) pptr[t + 1] += pptr[t];
         const uint64_t np = pptr[T];
         pvec<uint32_t> plist(np);
-        // distinct min(2, max_weight) values of each range in order of first appearance
-        std::vector<std::vector<float>> seen_w(T);
         run_threads(T, [&](unsigned t) {
             uint64_t q = pptr[t];
-            for (uint64_t j = e * t / T; j < e * (t + 1) / T; ++j) {
-                if (!d.plastic[j]) continue;
-                plist[q++] = (uint32_t)j;
-                // StdpRule::apply: maximum = std::min(config max 2.0f, synapse max_weight)
-                const float mx = (d.max_weight[j] < 2.0f) ? d.max_weight[j] : 2.0f;
-                auto& sw = seen_w[t];
-                if (sw.size() <= 256 &&
-                    std::none_of(sw.begin(), sw.end(), [&](float y) { return std::memcmp(&y, &mx, 4) == 0; }))
-                    sw.push_back(mx);
-            }
+            for (uint64_t j = e * t / T; j < e * (t + 1) / T; ++j)
+                if (d.plastic[j]) plist[q++] = (uint32_t)j;
         });
-        for (unsigned t = 0; t < T; ++t)
-            for (float mx : seen_w[t]) {
-                uint32_t idx = 0;
-                while (idx < h.wmax.size() && std::memcmp(&h.wmax[idx], &mx, 4) != 0) ++idx;
-                if (idx == h.wmax.size()) {
-                    if (idx == 256) throw std::invalid_argument("more than 256 distinct max_weight values on plastic synapses");
-                    h.wmax.push_back(mx);
-                }
-            }
         pvec<uint32_t> porder;
         stable_sort_by_key(np, (uint64_t)n, [&](uint64_t q) { return (uint64_t)d.target[plist[q]]; }, T, porder,
                            h.in_ptr);
@@ -483,14 +463,13 @@ inline HostNet build_host_net(const NetDesc& d, unsigned threads = 0) {
                 const uint32_t j = plist[porder[p]];
                 h.tpos[h.slot_of[j]] = (uint32_t)p;
                 h.tpos_conn[p] = j;
-                const float mx = (d.max_weight[j] < 2.0f) ? d.max_weight[j] : 2.0f;
-                uint32_t idx = 0;
-                while (idx < h.wmax.size() && std::memcmp(&h.wmax[idx], &mx, 4) != 0) ++idx;
                 StdpRec r{};
                 r.weight = d.weight[j];
                 r.last_pre = kNone;
                 r.last_post = kNone;
-                r.aux = idx;
+                r.aux = 0;
+                // StdpRule::apply: maximum = std::min(config max 2.0f, synapse max_weight)
+                r.hi = (d.max_weight[j] < 2.0f) ? d.max_weight[j] : 2.0f;
                 h.stdp[p] = r;
             }
         });

@@ -588,9 +588,8 @@ SNN_HD uint32_t code_observeN(CodeRecN& r, uint32_t tick, uint32_t times, uint32
 // ------------------------------------------------------------------ STDP
 
 // Plastic-synapse record, stored target-major so a post-synaptic scan is contiguous.
-//   aux bits 0-7: index of min(2.0f, max_weight) in the network's wmax table (StdpRule::apply's
-//   `maximum`); bits 8-31: post-synaptic updates already applied at tick last_post (only meaningful
-//   while last_post is the current tick).
+//   aux bits 8-31: post-synaptic updates already applied at tick last_post (only meaningful while
+//   last_post is the current tick); bits 0-7 zero.
 struct alignas(32) StdpRec {  // 32 B: a synapse's STDP state and counters in one cache-line access
     float weight;
     uint32_t last_pre;     // kNone when unset
@@ -598,7 +597,8 @@ struct alignas(32) StdpRec {  // 32 B: a synapse's STDP state and counters in on
     uint32_t aux;
     uint32_t upd;          // changed updates (plasticity_update_count)
     uint32_t ltp;          // of which with delta > 0
-    uint32_t pad[2];
+    float hi;              // min(2.0f, max_weight): StdpRule::apply's `maximum`
+    uint32_t pad;
 };
 
 // StdpRule::apply: clamp(w + delta, 0, min(2, max_w)), std::clamp written out as libstdc++ does.
@@ -621,8 +621,7 @@ struct StdpTally {
         r.ltp += ltp;
     }
 };
-SNN_HD uint32_t stdp_apply(StdpRec& r, uint32_t pre, uint32_t post, const float* ltp, const float* ltd,
-                           const float* wmax) {
+SNN_HD uint32_t stdp_apply(StdpRec& r, uint32_t pre, uint32_t post, const float* ltp, const float* ltd) {
     float delta = 0.0f;
     if (post > pre) {
         const uint32_t a = post - pre;
@@ -633,7 +632,7 @@ SNN_HD uint32_t stdp_apply(StdpRec& r, uint32_t pre, uint32_t post, const float*
     }
     const float before = r.weight;
     const float v = r.weight + delta;
-    const float hi = wmax[r.aux & 0xFFu];
+    const float hi = r.hi;
     r.weight = (v < 0.0f) ? 0.0f : (hi < v) ? hi : v;
     if ((r.weight - before) == 0.0f) return 0u;
     return delta > 0.0f ? kStdpLtp : kStdpLtd;

@@ -103,7 +103,6 @@ struct Step {
     const uint32_t* tpos;
     const uint64_t* in_ptr;
     StdpRec* stdp;
-    const float* wmax;           // wmax table indexed by StdpRec::aux bits 0-7
     const NeuronParams* np;
     uint32_t tp_uniform;         // every neuron has the same window / similarity / max_refs
     uint32_t tp_window, tp_max_refs;
@@ -453,9 +452,9 @@ SNN_HD StdpTally post_catch_up(const Step& S, StdpRec& r, uint32_t T, uint32_t f
     StdpTally changed;
     for (; applied < fires; ++applied) {
         r.last_post = T;
-        if (r.last_pre != kNone) changed.add(stdp_apply(r, r.last_pre, T, S.ltp, S.ltd, S.wmax));
+        if (r.last_pre != kNone) changed.add(stdp_apply(r, r.last_pre, T, S.ltp, S.ltd));
     }
-    if (r.last_post == T) r.aux = (r.aux & 0xFFu) | (applied << 8);
+    if (r.last_post == T) r.aux = applied << 8;
     return changed;
 }
 
@@ -511,7 +510,7 @@ SNN_HD uint32_t fold_piece(const Step& S, FoldTarget& f, E* ev, uint32_t kc, uin
                 if constexpr (CODE_HERE) dl = f2u(e.w);
                 else dl = S.delay[e.slot];
                 const uint32_t pre = T >= dl ? T - dl : T;
-                if (r.last_post != kNone) changed.add(stdp_apply(r, pre, r.last_post, S.ltp, S.ltd, S.wmax));
+                if (r.last_post != kNone) changed.add(stdp_apply(r, pre, r.last_post, S.ltp, S.ltd));
                 r.last_pre = pre;
                 changed.store(r);
                 S.stdp[e.tpos] = r;
@@ -641,7 +640,7 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
                 if constexpr (CODE_HERE) dl = f2u(e.w);
                 else dl = S.delay[e.slot];
                 const uint32_t pre = T >= dl ? T - dl : T;
-                if (r.last_post != kNone) changed.add(stdp_apply(r, pre, r.last_post, S.ltp, S.ltd, S.wmax));
+                if (r.last_post != kNone) changed.add(stdp_apply(r, pre, r.last_post, S.ltp, S.ltd));
                 r.last_pre = pre;
                 changed.store(r);
                 S.stdp[e.tpos] = r;

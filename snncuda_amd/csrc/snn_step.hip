@@ -1383,7 +1383,8 @@ __global__ void __launch_bounds__(kBlock) k_gen_place(GenParams P) {
                 r.weight = g.w;
                 r.last_pre = kNone;
                 r.last_post = kNone;
-                r.aux = 0;  // wmax table entry 0 = 2.0
+                r.aux = 0;
+                r.hi = 2.0f;  // max_weight 2.0 on every synapse: min(2, 2)
                 P.stdp[tp] = r;
                 P.tpos_conn[tp] = (uint32_t)(r0 + j);
             } else if (P.tpos) {
@@ -2142,7 +2143,6 @@ std::unique_ptr<snn_net> create_net(const NetDesc& d, const snn_options& o) {
         S.tpos = dev_upload(own, h.tpos);
         S.in_ptr = dev_upload(own, h.in_ptr);
         S.stdp = dev_upload(own, h.stdp);
-        S.wmax = dev_upload(own, h.wmax);
         net->tpos_conn_dev = dev_upload(own, h.tpos_conn);
         net->n_tpos = h.stdp.size();
     }
@@ -2273,8 +2273,6 @@ std::unique_ptr<snn_net> generate_nets(const snn_ei_desc* g, uint64_t m, const s
         gp.tpos_conn = dev_alloc<uint32_t>(own, n_tpos, false);
         S.tpos = gp.tpos;
         S.stdp = gp.stdp;
-        std::vector<float> wm(1, 2.0f);  // max_weight 2.0 on every synapse: min(2, 2)
-        S.wmax = dev_upload(own, wm);
         net->tpos_conn_dev = gp.tpos_conn;
         net->n_tpos = n_tpos;
     }

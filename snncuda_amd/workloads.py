@@ -252,11 +252,13 @@ def config0(seed: int = 0, threshold: float = 1.0, ticks: int = 1000, plastic: b
 
 def feature_net(seed: int, n: int = 60, fanout: int = 6, ticks: int = 300, n_codes: int = 3, code_max: int = 4,
                 delay_max: int = 6, plastic_frac: float = 0.7, threshold: float = 0.8, max_refs=(1, 2, 4),
-                window: int = 10, sim_thr: float = 0.9, interleave: bool = True) -> Net:
+                window: int = 10, sim_thr: float = 0.9, interleave: bool = True, max_weight_range=None) -> Net:
     """Small network exercising every feature of the step: 0..n_codes code offsets per synapse
     (including empty lists and duplicates), delay 0, all four compartments and receptors, mixed
     plasticity and max_refs, repeated fires per tick, and injects issued between run_until calls
-    (each inject's stage is the scheduler's next poll tick at the time of the call)."""
+    (each inject's stage is the scheduler's next poll tick at the time of the call).
+    max_weight_range=(lo, hi) draws every synapse's max_weight uniformly from [lo, hi) (one clamp
+    bound per synapse) from a generator of its own, so the other arrays are those of the default."""
     rng = np.random.default_rng(seed)
     e = n * fanout
     src = np.repeat(np.arange(n), fanout)
@@ -267,6 +269,8 @@ def feature_net(seed: int, n: int = 60, fanout: int = 6, ticks: int = 300, n_cod
     delay = rng.integers(0, delay_max + 1, e).astype(np.uint32)
     plastic = (rng.random(e) < plastic_frac).astype(np.uint8)
     max_w = rng.choice(np.array([0.5, 1.0, 2.0, 3.0], np.float32), e)
+    if max_weight_range is not None:
+        max_w = np.random.default_rng(seed + 7919).uniform(*max_weight_range, e).astype(np.float32)
     codes = [list(rng.integers(0, code_max + 1, rng.integers(0, n_codes + 1))) for _ in range(e)]
     nrn = neurons(n, threshold, window, sim_thr, 1)
     nrn[3][:] = rng.choice(np.asarray(max_refs, np.uint64), n)

@@ -292,7 +292,6 @@ void* emu_create(uint64_t n, const float* thr, const uint64_t* win, const float*
             S.tpos = h.tpos.data();
             S.in_ptr = h.in_ptr.data();
             S.stdp = h.stdp.data();
-            S.wmax = h.wmax.data();
         }
         S.np = h.params.data();
         S.tp_uniform = 1;
@@ -542,7 +541,7 @@ int emu_pack_check(uint64_t n, const float* thr, const uint64_t* win, const floa
                       same(a.row, b.row), same(a.grp, b.grp), same(a.ve, b.ve), same(a.tgt, b.tgt), same(a.w, b.w),
                       same(a.meta, b.meta), same(a.delay, b.delay), same(a.expected, b.expected),
                       same(a.slot_of, b.slot_of), same(a.tpos, b.tpos), same(a.in_ptr, b.in_ptr),
-                      same(a.stdp, b.stdp), same(a.tpos_conn, b.tpos_conn), same(a.wmax, b.wmax),
+                      same(a.stdp, b.stdp), same(a.tpos_conn, b.tpos_conn),
                       same(a.params, b.params)};
     for (size_t i = 0; i < sizeof f; ++i)
         if (!f[i]) return (int)i + 1;

@@ -163,18 +163,12 @@ inline HostNet build_host_net_serial(const NetDesc& d) {
             h.tpos[h.slot_of[j]] = (uint32_t)t;
             h.tpos_conn[t] = (uint32_t)j;
             // StdpRule::apply: maximum = std::min(config max 2.0f, synapse max_weight)
-            const float mx = (d.max_weight[j] < 2.0f) ? d.max_weight[j] : 2.0f;
-            uint32_t idx = 0;
-            while (idx < h.wmax.size() && std::memcmp(&h.wmax[idx], &mx, 4) != 0) ++idx;
-            if (idx == h.wmax.size()) {
-                if (idx == 256) throw std::invalid_argument("more than 256 distinct max_weight values on plastic synapses");
-                h.wmax.push_back(mx);
-            }
             StdpRec r{};
             r.weight = d.weight[j];
             r.last_pre = kNone;
             r.last_post = kNone;
-            r.aux = idx;
+            r.aux = 0;
+            r.hi = (d.max_weight[j] < 2.0f) ? d.max_weight[j] : 2.0f;
             h.stdp[t] = r;
         }
     }

@@ -22,6 +22,7 @@ from tests import kat_nets as K
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 CHECK = ROOT / "oracle" / "_ref" / "backend_check"
 EXPERIMENT = ROOT / "oracle" / "_ref" / "cuda_learning_consistency_experiment"
+THROUGHPUT = ROOT / "oracle" / "_ref" / "spike_throughput_benchmark"
 needs_check = pytest.mark.skipif(not CHECK.exists(), reason="oracle/_ref/backend_check not built")
 
 
@@ -80,6 +81,21 @@ def test_reference_learning_consistency_experiment_passes():
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
     assert "cuda_learning_consistency_experiment=passed" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not THROUGHPUT.exists(), reason="oracle/_ref benchmark not built")
+def test_reference_spike_throughput_benchmark_passes():
+    # the reference's tests/spike_throughput_benchmark.cpp (27,648-neuron chain net, 6 ticks) on the
+    # drop-in: it exits 0 only when the device's delivered and fired counts equal the reference CPU's
+    print("reference tests/spike_throughput_benchmark.cpp on the drop-in backend", flush=True)
+    r = subprocess.run([str(THROUGHPUT)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    kv = dict(line.split("=", 1) for line in r.stdout.splitlines() if "=" in line)
+    assert kv["cuda_executed"] == "yes"
+    assert kv["cuda_delivered_spikes"] == kv["cpu_delivered_spikes"] and int(kv["cpu_delivered_spikes"]) > 0
+    assert kv["cuda_fired_spikes"] == kv["cpu_fired_spikes"]
 
 
 @pytest.mark.gpu

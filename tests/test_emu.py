@@ -81,3 +81,23 @@ def test_network_too_large_for_one_device_is_refused_at_create():
     assert partition_shift(8192 * 4096)[0] == 12
     r, msg = partition_shift(8192 * 4096 + 1)
     assert r == -1 and "8192" in msg and "target partitions" in msg
+
+
+def test_a_clamp_bound_per_plastic_synapse_matches_the_oracle():
+    """StdpRule::apply clamps each synapse to min(2, its own max_weight) (src/learning/STDP.cpp:37-40):
+    thousands of distinct bounds, most of them below the initial weights so the clamp acts."""
+    import oracle
+    from snncuda_amd import workloads as W
+    net = W.feature_net(21, n=2000, ticks=600, plastic_frac=0.9, threshold=0.6, max_weight_range=(0.05, 0.6))
+    assert np.unique(net.max_weight[net.plastic != 0]).size > 1000
+    ref = oracle.Oracle(net)
+    ref.run_commands(net.commands)
+    m = Emu(net, ev_cap=1 << 22, fire_cap=1 << 20)
+    m.run_commands(net.commands)
+    assert m.flags == 0
+    rs, ms = ref.synapses(), m.synapses()
+    assert int(rs["upd_count"].sum()) > 1000
+    at_bound = (net.plastic != 0) & (rs["upd_count"] > 0) & (rs["weight"] == np.minimum(net.max_weight, 2.0))
+    assert np.unique(net.max_weight[at_bound]).size > 200
+    assert np.array_equal(ms["weight"], rs["weight"])
+    assert np.array_equal(m.raster(), ref.fires()[1])

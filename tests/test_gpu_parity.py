@@ -141,3 +141,13 @@ def test_bench_law_matches_oracle(stdp):
         ds, rs = dev.synapses(), ref.synapses()
         for k in ("weight", "upd_count", "last_pre", "last_post", "c_match", "c_learned"):
             assert np.array_equal(ds[k], rs[k].astype(ds[k].dtype)), k
+
+
+def test_a_clamp_bound_per_plastic_synapse_matches_the_oracle():
+    net = W.feature_net(21, n=2000, ticks=600, plastic_frac=0.9, threshold=0.6, max_weight_range=(0.05, 0.6))
+    print("feature_net(21, n=2000) with a max_weight per synapse", flush=True)
+    ref = oracle.Oracle(net)
+    ref.run_commands(net.commands)
+    with run_device(net) as dev:
+        assert np.array_equal(dev.synapses()["weight"], ref.synapses()["weight"])
+        assert np.array_equal(dev.raster(), ref.fires()[1])
