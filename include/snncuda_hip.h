@@ -66,6 +66,10 @@ typedef struct snn_options {
     uint64_t raster_capacity;       /* fires recorded for snn_net_read_raster over the handle's life */
     int32_t track_pre_count;        /* keep per-synapse delivered counts (pre_spike_count) */
     int32_t device;                 /* HIP device ordinal */
+    uint64_t learned_pattern_slots; /* learned temporal patterns held per neuron (0: the network's
+                                       largest max_reference_patterns, as many as a quarter of the
+                                       device's memory holds); a neuron whose max_reference_patterns
+                                       is larger fails the run once it holds that many */
 } snn_options;
 
 typedef struct snn_net snn_net;

@@ -39,7 +39,8 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;      // unset tick (std::nullopt)
 constexpr uint32_t kDecayTable = 16384;      // expf(-dt/tau) is exactly 0 beyond this for tau <= 150
 constexpr uint32_t kStdpTable = 4096;        // expf(-dt/20) is exactly 0 beyond this
 constexpr uint32_t kRuns = 8;                // run-length capacity of a neuron temporal pattern
-constexpr uint32_t kNeuronRefs = 4;          // learned-pattern capacity per neuron
+constexpr uint32_t kHeadMask = 0x00FFFFFFu;  // TemporalHdr::head_last: learned-pattern ring head
+constexpr uint32_t kLastMatch = 1u << 24;     // TemporalHdr::head_last: the last observation matched
 constexpr uint32_t kCodeMax = 8;             // max offsets in a synapse code pattern
 constexpr uint32_t kCodeRefs = 4;            // learned code-pattern capacity (general record)
 
@@ -93,8 +94,8 @@ struct TemporalHdr {       // 32 B
     uint32_t maxlen;       // longest learned pattern (0 when none)
     uint32_t nruns;        // stored runs of the candidate
     uint32_t trunc;        // 1: candidate has more runs than stored
-    uint32_t nlearned;     // learned patterns held (<= kNeuronRefs)
-    uint32_t head_last;    // bits 0-7 ring head, bit 8 last_match
+    uint32_t nlearned;     // learned patterns held (<= the ring's slots)
+    uint32_t head_last;    // bits 0-23 ring head (kHeadMask), bit 24 last_match (kLastMatch)
 };
 
 struct Runs {              // 64 B, run-length encoded non-decreasing offsets
@@ -102,10 +103,18 @@ struct Runs {              // 64 B, run-length encoded non-decreasing offsets
     uint32_t cnt[kRuns];
 };
 
-struct TemporalRefs {      // learned patterns as a FIFO ring
-    Runs ref[kNeuronRefs];
-    uint32_t len[kNeuronRefs];
-    uint32_t nruns[kNeuronRefs];
+struct RefMeta {           // a learned pattern's length and stored runs
+    uint32_t len;
+    uint32_t nruns;
+};
+
+// One neuron's learned patterns (learn_pattern's FIFO of at most max_reference_patterns copies,
+// SynapseDendriteProcessor.cpp:31-41) as a ring of `cap` slots in the network's pattern pool:
+// neuron x owns slots [x * cap, (x + 1) * cap) of the Runs and RefMeta arrays.
+struct RefsView {
+    Runs* ref;
+    RefMeta* meta;
+    uint32_t cap;
 };
 
 // cosine over run-length encoded sequences, prefix of k = min(len_l, len_r) entries.
@@ -178,8 +187,8 @@ SNN_HD uint32_t stored_len(const Runs& c, uint32_t nruns) {
 // One observation of update_temporal_pattern (src/runtime/SynapseDendriteProcessor.cpp:294-316)
 // with matches_learned_pattern (:14-29) and learn_pattern (:31-41). `cand` and `refs` are only
 // dereferenced when needed; they may point to global memory. Returns error flags.
-template <typename CandPtr, typename RefsPtr>
-SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, RefsPtr refs, uint32_t tick,
+template <typename CandPtr>
+SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, const RefsView& refs, uint32_t tick,
                                  const NeuronParams& p, bool& cand_dirty) {
     uint32_t err = 0;
     const bool reset = h.n == 0 || tick - h.start > p.window;
@@ -190,7 +199,7 @@ SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, RefsPtr refs, ui
         h.trunc = 0;
     }
     const uint32_t v = tick - h.start;
-    const bool last_match = (h.head_last >> 8) & 1u;
+    const bool last_match = (h.head_last & kLastMatch) != 0;
     // With every learned pattern no longer than the offsets seen before this one, the compared
     // prefixes are unchanged, so the previous (matching) result repeats.
     const bool repeat = !reset && last_match && h.n >= h.maxlen;
@@ -218,48 +227,48 @@ SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, RefsPtr refs, ui
         // every compared prefix must be inside the stored runs
         const uint32_t have = h.trunc ? stored_len(*cand, h.nruns) : h.n;
         float best = 0.0f;
-        const uint32_t head = h.head_last & 0xFFu;
+        const uint32_t head = h.head_last & kHeadMask;
         for (uint32_t q = 0; q < h.nlearned; ++q) {
-            const uint32_t slot = (head + q) % kNeuronRefs;
-            const uint32_t rl = refs->len[slot];
+            const uint32_t slot = (head + q) % refs.cap;
+            const uint32_t rl = refs.meta[slot].len;
             const uint32_t need = rl < h.n ? rl : h.n;
             if (need > have) {
                 err |= kErrTemporal;
                 continue;
             }
-            const float c = cosine_runs(refs->ref[slot], refs->nruns[slot], rl, *cand, h.nruns, need);
+            const float c = cosine_runs(refs.ref[slot], refs.meta[slot].nruns, rl, *cand, h.nruns, need);
             best = (best < c) ? c : best;
         }
         match = best >= p.sim_thr;
     }
     if (match) {
         h.match += 1;
-        h.head_last |= 1u << 8;
+        h.head_last |= kLastMatch;
         return err;
     }
-    h.head_last &= ~(1u << 8);
+    h.head_last &= ~kLastMatch;
     if (h.n < 2) return err;
     // learn a copy of the candidate (FIFO eviction at max_refs)
     if (h.trunc) return err | kErrTemporal;
-    uint32_t head = h.head_last & 0xFFu;
+    uint32_t head = h.head_last & kHeadMask;
     if (h.nlearned >= p.max_refs) {
-        head = (head + 1) % kNeuronRefs;
+        head = (head + 1) % refs.cap;
         h.nlearned -= 1;
     }
-    if (h.nlearned >= kNeuronRefs) return err | kErrTemporal;
-    const uint32_t slot = (head + h.nlearned) % kNeuronRefs;
+    if (h.nlearned >= refs.cap) return err | kErrTemporal;
+    const uint32_t slot = (head + h.nlearned) % refs.cap;
     for (uint32_t i = 0; i < h.nruns; ++i) {
-        refs->ref[slot].val[i] = cand->val[i];
-        refs->ref[slot].cnt[i] = cand->cnt[i];
+        refs.ref[slot].val[i] = cand->val[i];
+        refs.ref[slot].cnt[i] = cand->cnt[i];
     }
-    refs->len[slot] = h.n;
-    refs->nruns[slot] = h.nruns;
+    refs.meta[slot].len = h.n;
+    refs.meta[slot].nruns = h.nruns;
     h.nlearned += 1;
-    h.head_last = (h.head_last & ~0xFFu) | head;
+    h.head_last = (h.head_last & ~kHeadMask) | head;
     uint32_t ml = 0;
     for (uint32_t q = 0; q < h.nlearned; ++q) {
-        const uint32_t s = (head + q) % kNeuronRefs;
-        ml = refs->len[s] > ml ? refs->len[s] : ml;
+        const uint32_t s = (head + q) % refs.cap;
+        ml = refs.meta[s].len > ml ? refs.meta[s].len : ml;
     }
     h.maxlen = ml;
     return err;
@@ -274,65 +283,58 @@ SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, RefsPtr refs, ui
 //
 // A tick that restarts the window appends offset 0 `times` times. Every compared candidate prefix is
 // then all zeros, so every cosine is 0 (TemporalPattern.cpp:51-52) and, with a similarity threshold
-// above 0, no observation matches: the first (n = 1) only clears last_match and each later one
-// learns a copy of the candidate, a single run of n zeros. temporal_restart_tick applies those
-// learns to the ring in registers and writes each touched slot once.
-template <typename CandPtr, typename RefsPtr>
-SNN_HD uint32_t temporal_restart_tick(TemporalHdr& h, CandPtr cand, RefsPtr refs, uint32_t times,
+// above 0, no observation matches: the first (n = 1) only clears last_match and learn j = 0, 1, ...
+// (observation n = j + 2) learns a copy of the candidate, a single run of j + 2 zeros.
+// temporal_restart_tick places those learns in closed form. With max_refs M no larger than the ring:
+// learn j first evicts the oldest pattern when nl0 + j >= M, so it lands at ring position
+// min(nl0 + j, M - 1) after max(0, nl0 + j + 1 - M) evictions, and a learn followed by M others is
+// evicted again, so only the last M are written (slots outside the ring are never read). With M
+// above the ring's slots nothing is evicted and the learns beyond the free slots are capacity errors.
+template <typename CandPtr>
+SNN_HD uint32_t temporal_restart_tick(TemporalHdr& h, CandPtr cand, const RefsView& refs, uint32_t times,
                                       const NeuronParams& p) {
     cand->val[0] = 0;
     cand->cnt[0] = times;
     h.nruns = 1;
     h.n = times;
-    h.head_last &= ~(1u << 8);
-    uint32_t err = 0;
-    uint32_t head = h.head_last & 0xFFu, nl = h.nlearned;
-    uint32_t len[kNeuronRefs] = {0, 0, 0, 0};
-    uint32_t written = 0;
-    bool learned = false;
-    for (uint32_t n = 2; n <= times; ++n) {
-        uint32_t hd = head;
-        if (nl >= p.max_refs) {
-            hd = (hd + 1) % kNeuronRefs;
-            nl -= 1;
+    h.head_last &= ~kLastMatch;
+    if (times < 2) return 0;
+    const uint32_t L = times - 1, C = refs.cap, M = p.max_refs, nl0 = h.nlearned;
+    const uint32_t head0 = h.head_last & kHeadMask;
+    uint32_t err = 0, head = head0, nl;
+    auto learn = [&](uint32_t slot, uint32_t len) {
+        refs.ref[slot].val[0] = 0;
+        refs.ref[slot].cnt[0] = len;
+        refs.meta[slot].len = len;
+        refs.meta[slot].nruns = 1;
+    };
+    if (M <= C) {
+        for (uint32_t j = L > M ? L - M : 0; j < L; ++j) {
+            const uint32_t ev = nl0 + j + 1 > M ? nl0 + j + 1 - M : 0;
+            const uint32_t pos = nl0 + j < M - 1 ? nl0 + j : M - 1;
+            learn((head0 + ev + pos) % C, j + 2);
         }
-        if (nl >= kNeuronRefs) {
-            err |= kErrTemporal;
-            continue;
-        }
-        const uint32_t slot = (hd + nl) % kNeuronRefs;
-#pragma unroll
-        for (uint32_t q = 0; q < kNeuronRefs; ++q) len[q] = q == slot ? n : len[q];
-        written |= 1u << slot;
-        nl += 1;
-        head = hd;
-        learned = true;
+        head = (head0 + (nl0 + L > M ? nl0 + L - M : 0)) % C;
+        nl = nl0 + L < M ? nl0 + L : M;
+    } else {
+        const uint32_t room = C - nl0, k = L < room ? L : room;
+        for (uint32_t j = 0; j < k; ++j) learn((head0 + nl0 + j) % C, j + 2);
+        if (L > room) err |= kErrTemporal;
+        nl = nl0 + k;
     }
     h.nlearned = nl;
-    if (!learned) return err;
-    h.head_last = (h.head_last & ~0xFFu) | head;
+    h.head_last = (h.head_last & ~kHeadMask) | head;
     uint32_t ml = 0;
-#pragma unroll
-    for (uint32_t q = 0; q < kNeuronRefs; ++q) {
-        if (written & (1u << q)) {
-            refs->ref[q].val[0] = 0;
-            refs->ref[q].cnt[0] = len[q];
-            refs->len[q] = len[q];
-            refs->nruns[q] = 1;
-        }
-        // q's place in the ring after head
-        const uint32_t pos = (q + kNeuronRefs - head) % kNeuronRefs;
-        if (pos < nl) {
-            const uint32_t l = (written & (1u << q)) ? len[q] : refs->len[q];
-            ml = l > ml ? l : ml;
-        }
+    for (uint32_t q = 0; q < nl; ++q) {
+        const uint32_t l = refs.meta[(head + q) % C].len;
+        ml = l > ml ? l : ml;
     }
     h.maxlen = ml;
     return err;
 }
 
-template <typename CandPtr, typename RefsPtr>
-SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, RefsPtr refs, uint32_t tick, uint32_t times,
+template <typename CandPtr>
+SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, const RefsView& refs, uint32_t tick, uint32_t times,
                                       const NeuronParams& p) {
     uint32_t err = 0;
     if (times == 0) return 0;
@@ -361,7 +363,7 @@ SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, RefsPtr refs
         }
     }
     for (uint32_t t = 0; t < times; ++t) {
-        const bool last_match = (h.head_last >> 8) & 1u;
+        const bool last_match = (h.head_last & kLastMatch) != 0;
         const bool repeat = !(reset && t == 0) && last_match && h.n >= h.maxlen;
         if (repeat) {
             // this and every later observation of the tick match
@@ -377,26 +379,26 @@ SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, RefsPtr refs
         if (h.nlearned != 0 && h.n >= 2) {
             const uint32_t have = h.trunc ? stored_len(*cand, h.nruns) : h.n;
             float best = 0.0f;
-            const uint32_t head = h.head_last & 0xFFu;
+            const uint32_t head = h.head_last & kHeadMask;
             for (uint32_t q = 0; q < h.nlearned; ++q) {
-                const uint32_t slot = (head + q) % kNeuronRefs;
-                const uint32_t rl = refs->len[slot];
+                const uint32_t slot = (head + q) % refs.cap;
+                const uint32_t rl = refs.meta[slot].len;
                 const uint32_t need = rl < h.n ? rl : h.n;
                 if (need > have) {
                     err |= kErrTemporal;
                     continue;
                 }
-                const float c = cosine_runs_ov(refs->ref[slot], rl, *cand, need, run, cnt);
+                const float c = cosine_runs_ov(refs.ref[slot], rl, *cand, need, run, cnt);
                 best = (best < c) ? c : best;
             }
             match = best >= p.sim_thr;
         }
         if (match) {
             h.match += 1;
-            h.head_last |= 1u << 8;
+            h.head_last |= kLastMatch;
             continue;
         }
-        h.head_last &= ~(1u << 8);
+        h.head_last &= ~kLastMatch;
         if (h.n < 2) continue;
         // learn a copy of the candidate (FIFO eviction at max_refs)
         if (h.trunc) {
@@ -404,28 +406,28 @@ SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, RefsPtr refs
             continue;
         }
         if (run < kRuns) cand->cnt[run] = cnt;
-        uint32_t head = h.head_last & 0xFFu;
+        uint32_t head = h.head_last & kHeadMask;
         if (h.nlearned >= p.max_refs) {
-            head = (head + 1) % kNeuronRefs;
+            head = (head + 1) % refs.cap;
             h.nlearned -= 1;
         }
-        if (h.nlearned >= kNeuronRefs) {
+        if (h.nlearned >= refs.cap) {
             err |= kErrTemporal;
             continue;
         }
-        const uint32_t slot = (head + h.nlearned) % kNeuronRefs;
+        const uint32_t slot = (head + h.nlearned) % refs.cap;
         for (uint32_t i = 0; i < h.nruns; ++i) {
-            refs->ref[slot].val[i] = cand->val[i];
-            refs->ref[slot].cnt[i] = cand->cnt[i];
+            refs.ref[slot].val[i] = cand->val[i];
+            refs.ref[slot].cnt[i] = cand->cnt[i];
         }
-        refs->len[slot] = h.n;
-        refs->nruns[slot] = h.nruns;
+        refs.meta[slot].len = h.n;
+        refs.meta[slot].nruns = h.nruns;
         h.nlearned += 1;
-        h.head_last = (h.head_last & ~0xFFu) | head;
+        h.head_last = (h.head_last & ~kHeadMask) | head;
         uint32_t ml = 0;
         for (uint32_t q = 0; q < h.nlearned; ++q) {
-            const uint32_t s2 = (head + q) % kNeuronRefs;
-            ml = refs->len[s2] > ml ? refs->len[s2] : ml;
+            const uint32_t s2 = (head + q) % refs.cap;
+            ml = refs.meta[s2].len > ml ? refs.meta[s2].len : ml;
         }
         h.maxlen = ml;
     }

@@ -110,7 +110,9 @@ struct Step {
     NeuronDyn* dyn;
     TemporalHdr* th;
     Runs* cand;
-    TemporalRefs* refs;
+    Runs* ref_runs;              // learned patterns: ref_cap slots per neuron
+    RefMeta* ref_meta;
+    uint32_t ref_cap;
     uint32_t* last_fired;
     const float* decay_tab;      // 4 x kDecayTable
     const float* ltp;            // kStdpTable
@@ -169,6 +171,11 @@ struct Step {
     uint32_t* fire_evt;          // optional: per fire position g, the index of the firing event in the
     uint32_t* raster_evt;        // target's ordered events of the tick; and per raster entry
 };
+
+SNN_HD RefsView refs_of(const Step& S, uint32_t x) {
+    const uint64_t b = (uint64_t)x * S.ref_cap;
+    return RefsView{S.ref_runs + b, S.ref_meta + b, S.ref_cap};
+}
 
 SNN_HD uint32_t lower_bound_u32(const uint32_t* a, uint64_t lo, uint64_t hi, uint32_t v) {
     while (lo < hi) {
@@ -583,7 +590,7 @@ SNN_HD uint32_t fold_end_apply(const Step& S, const FoldTarget& f, uint32_t x, u
         p = S.np[x];
     }
     p.threshold = f.threshold;
-    const uint32_t err = temporal_observe_tick(h, &S.cand[x], &S.refs[x], T, f.k, p);
+    const uint32_t err = temporal_observe_tick(h, &S.cand[x], refs_of(S, x), T, f.k, p);
     const uint32_t fires = f.taus_fires >> 8;
     if (fires) {
         S.fire_mult[f.first_g] = fires;
@@ -682,7 +689,7 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
     }
     // the neuron temporal pattern does not feed the membrane: its k observations of this tick are
     // applied together
-    err |= temporal_observe_tick(h, cand, &S.refs[x], T, k, p);
+    err |= temporal_observe_tick(h, cand, refs_of(S, x), T, k, p);
     if (fires) {
         S.fire_mult[first_g] = fires;
         S.last_fired[x] = T;

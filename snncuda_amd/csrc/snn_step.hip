@@ -2000,7 +2000,6 @@ void open_net(snn_net* net, const snn_options& o, uint32_t n, uint64_t e, uint32
     S.dyn = dev_alloc<NeuronDyn>(own, n);
     S.th = dev_alloc<TemporalHdr>(own, n);
     S.cand = dev_alloc<Runs>(own, n);
-    S.refs = dev_alloc<TemporalRefs>(own, n);
     S.last_fired = dev_alloc<uint32_t>(own, n, false);
     check(hipMemset(S.last_fired, 0xFF, (size_t)n * 4), "hipMemset");
     S.decay_tab = dev_upload(own, host_decay_table());
@@ -2025,6 +2024,21 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     S.tp_window = params[0].window;
     S.tp_max_refs = params[0].max_refs;
     S.tp_sim = params[0].sim_thr;
+    // learned-pattern pool: as many slots per neuron as the largest max_reference_patterns (or the
+    // option), within a quarter of the device's memory; a neuron that learns beyond its slots
+    // reports SNN_ECAPACITY
+    {
+        uint64_t want = 1;
+        for (const auto& p : params) want = std::max<uint64_t>(want, p.max_refs);
+        if (o.learned_pattern_slots) want = std::min<uint64_t>(want, o.learned_pattern_slots);
+        size_t free_b = 0, total_b = 0;
+        check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const uint64_t per = sizeof(Runs) + sizeof(RefMeta);
+        const uint64_t fit = std::max<uint64_t>(1, total_b / 4 / ((uint64_t)std::max<uint32_t>(n, 1) * per));
+        S.ref_cap = (uint32_t)std::min<uint64_t>({want, fit, (uint64_t)kHeadMask});
+        S.ref_runs = dev_alloc<Runs>(own, (uint64_t)n * S.ref_cap);
+        S.ref_meta = dev_alloc<RefMeta>(own, (uint64_t)n * S.ref_cap);
+    }
     // code pattern records: per source when every synapse has one code offset and every target the
     // same temporal parameters (SNNCUDA_CODE_PER_SYNAPSE=1 keeps them per synapse), else per synapse
     const char* eager = std::getenv("SNNCUDA_CODE_PER_SYNAPSE");

@@ -53,22 +53,23 @@ def test_per_target_sort_orders_by_g_at_every_size(narrow_g, width, small):
 
 
 @pytest.mark.parametrize("sim", [0.9, 1.0, 0.5, 0.0, -0.5])
-@pytest.mark.parametrize("max_refs", [1, 2, 4, 5])
-def test_per_tick_temporal_update_equals_single_observations(sim, max_refs):
+@pytest.mark.parametrize("max_refs,cap", [(1, 1), (1, 4), (2, 4), (4, 4), (5, 4), (3, 7), (9, 9), (40, 64),
+                                          (500, 500), (500, 37)])
+def test_per_tick_temporal_update_equals_single_observations(sim, max_refs, cap):
     # The fold applies one neuron's k observations of a tick together (a window restart in closed
     # form, repeated matches in bulk); it must leave the header, candidate, learned ring and error
     # flags exactly as k calls of the one-observation update. Sparse and dense ticks, long gaps
-    # (window restarts), thresholds at and below 0 (every cosine of a restart matches) and ring
-    # capacities below, at and above the four slots (the error path).
+    # (window restarts), thresholds at and below 0 (every cosine of a restart matches), and
+    # max_reference_patterns below, at and above the ring's slots (above: the error path).
     from tests.emu import temporal_check
-    rng = np.random.default_rng(max_refs * 10 + int(sim * 10))
+    rng = np.random.default_rng(max_refs * 1000 + cap * 10 + int(sim * 10))
     for trial in range(40):
         window = int(rng.choice([1, 3, 8, 32]))
         m = 300
         gaps = rng.choice([1, 1, 1, 2, 5, window + 1, window + 7], size=m)
         ticks = np.cumsum(gaps).astype(np.uint32)
         times = rng.choice([1, 1, 2, 3, 7, 12, 30], size=m).astype(np.uint32)
-        assert temporal_check(ticks, times, window, sim, max_refs) == m, (trial, window)
+        assert temporal_check(ticks, times, window, sim, max_refs, cap) == m, (trial, window)
 
 
 def test_network_too_large_for_one_device_is_refused_at_create():
@@ -100,4 +101,47 @@ def test_a_clamp_bound_per_plastic_synapse_matches_the_oracle():
     at_bound = (net.plastic != 0) & (rs["upd_count"] > 0) & (rs["weight"] == np.minimum(net.max_weight, 2.0))
     assert np.unique(net.max_weight[at_bound]).size > 200
     assert np.array_equal(ms["weight"], rs["weight"])
+    assert np.array_equal(m.raster(), ref.fires()[1])
+
+
+@pytest.mark.parametrize("case,limit,under,flag", [
+    ("candidate_runs", {}, {"distinct_ticks": 8}, 8),      # 8 stored runs of a candidate
+    ("learned_patterns", {"ref_cap": 4}, {"max_refs": 4}, 8),  # the neuron's learned-pattern slots
+    ("code_patterns", {}, {"fires": 5}, 16),                # 4 patterns of a synapse's code record
+])
+def test_pattern_capacity_overflow_is_an_error_and_just_under_it_equals_the_oracle(case, limit, under, flag):
+    import oracle
+    from tests import capacity_nets as C
+    net = getattr(C, case)()
+    m = Emu(net, **limit)
+    m.run_commands(net.commands)
+    assert m.flags & flag, m.flags
+    net = getattr(C, case)(**under)
+    m = Emu(net, **limit)
+    m.run_commands(net.commands)
+    assert m.flags == 0
+    ref = oracle.Oracle(net)
+    ref.run_commands(net.commands)
+    for k in ("t_match", "t_learned", "membrane"):
+        assert np.array_equal(m.neurons()[k], ref.neurons()[k]), k
+    assert np.array_equal(m.synapses()["c_learned"], ref.synapses()["c_learned"])
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_neurons_holding_many_learned_patterns_match_the_oracle(seed):
+    # learn_pattern keeps up to max_reference_patterns copies (default 500, NetworkIR.h:18); the ring
+    # holds as many slots as the largest of them
+    import oracle
+    from tests import capacity_nets as C
+    net = C.many_learned(seed)
+    ref = oracle.Oracle(net)
+    ref.run_commands(net.commands)
+    rn = ref.neurons()
+    assert rn["t_learned"].max() > 16
+    m = Emu(net)
+    m.run_commands(net.commands)
+    assert m.flags == 0
+    mn = m.neurons()
+    for k in ("membrane", "current", "spike_count", "t_match", "t_learned"):
+        assert np.array_equal(mn[k], rn[k]), k
     assert np.array_equal(m.raster(), ref.fires()[1])

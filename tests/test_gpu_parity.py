@@ -151,3 +151,37 @@ def test_a_clamp_bound_per_plastic_synapse_matches_the_oracle():
     with run_device(net) as dev:
         assert np.array_equal(dev.synapses()["weight"], ref.synapses()["weight"])
         assert np.array_equal(dev.raster(), ref.fires()[1])
+
+
+@pytest.mark.parametrize("case,limit,under,name", [
+    ("candidate_runs", {}, {"distinct_ticks": 8}, "neuron-temporal-pattern"),
+    ("learned_patterns", {"learned_pattern_slots": 4}, {"max_refs": 4}, "neuron-temporal-pattern"),
+    ("code_patterns", {}, {"fires": 5}, "synapse-code-pattern"),
+])
+def test_pattern_capacity_overflow_raises_and_just_under_it_equals_the_oracle(case, limit, under, name):
+    from tests import capacity_nets as C
+    print(f"capacity case {case}", flush=True)
+    net = getattr(C, case)()
+    with pytest.raises(sa.CapacityError, match=name):
+        run_device(net, **limit)
+    net = getattr(C, case)(**under)
+    ref = oracle.Oracle(net)
+    ref.run_commands(net.commands)
+    with run_device(net, **limit) as dev:
+        for k in ("t_match", "t_learned", "membrane"):
+            assert np.array_equal(dev.neurons()[k], ref.neurons()[k]), k
+        assert np.array_equal(dev.synapses()["c_learned"], ref.synapses()["c_learned"].astype(np.uint64))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_neurons_holding_many_learned_patterns_match_the_oracle(seed):
+    from tests import capacity_nets as C
+    print(f"many_learned({seed})", flush=True)
+    net = C.many_learned(seed)
+    ref = oracle.Oracle(net)
+    ref.run_commands(net.commands)
+    with run_device(net) as dev:
+        dn, rn = dev.neurons(), ref.neurons()
+        for k in ("membrane", "current", "spike_count", "t_match", "t_learned", "last_fired"):
+            assert np.array_equal(dn[k], rn[k]), k
+        assert np.array_equal(dev.raster(), ref.fires()[1])
