@@ -25,7 +25,9 @@ The JSON line carries:
   scatter_roofline, step_roofline   the partition scatter, and the whole tick (same byte model / tick time).
   stdp_on        the same measurement with STDP on (excitatory synapses plastic, threshold 2.75).
   forms          the batched form (512 uneven networks stepped together vs looping the plain step
-                 over a sample of them) and the fixed-spike adjoint, measured in this run.
+                 over a sample of them), the fixed-spike adjoint, and the spike-code network (100,000
+                 neurons, 4 code offsets per synapse, STDP on; its whole 10,000-tick run), measured in
+                 this run.
   cpu_baseline   the oracle (plain-C restatement of the reference CPU step), one thread on this host,
                  on the same law at 20,000 neurons; cpu_baselines: the oracle and the reference's own
                  runtime (oracle/_ref/ref_harness, when built) on the other samples.
@@ -64,6 +66,11 @@ FOLD_BYTES_PER_NEURON = 4.0 + 2 * 32.0 + 2 * 32.0 + 8.0
 # with STDP the bucket record is 12 + 2 B and a plastic event reads and writes its 32-B STDP record
 FOLD_STDP_BYTES_PER_EVENT = 14.0
 FOLD_STDP_RECORD_BYTES = 2 * 32.0
+# spike-code network: several code offsets per synapse keep one 96-B code record per synapse (offsets
+# observed and learned), read and written by every event of the synapse, in place of the 32 B of the
+# per-source record; STDP on every synapse
+SPIKECODE_BYTES_PER_EVENT = 10.0 + 8.0 + 2 * 96.0 + STDP_EVENT_BYTES
+SPIKECODE_THRESHOLD = 4.6
 
 
 def parse():
@@ -72,10 +79,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=200, help="timed ticks")
     ap.add_argument("--warmup", type=int, default=50, help="untimed ticks after the settle phase")
     ap.add_argument("--neurons", type=int, default=1_000_000)
-    ap.add_argument("--form", choices=("all", "plain", "stdp", "batched", "adjoint"), default="all",
+    ap.add_argument("--form", choices=("all", "plain", "stdp", "batched", "adjoint", "spikecode"), default="all",
                     help="all (default): the plain step as the headline line with STDP on, the batched form, the "
-                         "adjoint and the CPU baselines measured in the same run; plain / stdp: one plain-step "
-                         "line; batched / adjoint: one line of that form")
+                         "adjoint, the spike-code network and the CPU baselines measured in the same run; plain / "
+                         "stdp: one plain-step line; batched / adjoint / spikecode: one line of that form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true",
                     help="one more pass of --steps ticks with HIP events around every kernel (kernel_ms_per_tick)")
@@ -85,6 +92,7 @@ def parse():
     ap.add_argument("--cpu-ticks", type=int, default=200)
     ap.add_argument("--nets", type=int, default=512, help="batched form: networks")
     ap.add_argument("--batch-ticks", type=int, default=2000, help="batched form: ticks per network")
+    ap.add_argument("--spikecode-ticks", type=int, default=10_000, help="spike-code form: ticks of the run")
     ap.add_argument("--loop-nets", type=int, default=16,
                     help="batched form: networks in the looped baseline, every (nets/loop-nets)-th (0: all)")
     return ap.parse_args()
@@ -324,6 +332,79 @@ def batched(args) -> dict:
     }
 
 
+# --------------------------------------------------------------------------- spike-code network
+
+def spikecode(args) -> dict:
+    """The spike-code network at full size: 100,000 neurons, fan-out 1,000 (1.01e8 synapses with the
+    1,000 input neurons'), 80/20 E/I, delays U{1..20}, 4 compartments, 4 code offsets per synapse from
+    U{0..7}, STDP on every synapse; the input neurons replay 10 fixed 8-spike / 32-tick patterns with
+    +-1 jitter over 5 Hz background. Pinned threshold 4.6: from 4.6 to 4.9 the network holds
+    ~0.5-1 recurrent fires per tick over the ~6 input fires for all 10,000 ticks; at 4.5 and below a
+    fluctuation ignites every neuron within ~1,000 ticks, and from 5.0 only the input neurons fire.
+    The whole run is timed (one run_until after --warmup untimed ticks, at most 100); value =
+    synaptic events/s. The fold (per-event code patterns, STDP) is the dominant kernel."""
+    import numpy as np
+    import snncuda_amd as sa
+    from snncuda_amd import workloads as W
+
+    T = args.spikecode_ticks
+    first = min(args.warmup, 100)
+    spec, cmds = W.config1_spec(0, ticks=T, threshold=SPIKECODE_THRESHOLD)
+    injects = [c for c in cmds if c[0] == 0]
+    inj_ticks = np.array([t for _, _, t, _ in injects], dtype=np.int64)
+    t_gen = time.perf_counter()
+    with sa.Network(spec=spec, max_events_per_tick=1 << 24, fire_log_capacity=1 << 22, raster_capacity=1 << 22,
+                    track_pre_count=False) as dev:
+        t_gen = time.perf_counter() - t_gen
+        dev.run_commands(injects)
+        if first:
+            dev.run_until(first - 1)
+        dev.set_profiling(True, kernels=("scatter", "fold"))
+        d0 = dev.delivered
+        t0 = time.perf_counter()
+        dev.run_until(T - 1)
+        elapsed = time.perf_counter() - t0
+        events = dev.delivered - d0 - int((inj_ticks >= first).sum())
+        kt = dev.kernel_times()
+        dpt, fpt = dev.tick_counts(first, T - first)
+        dev.set_profiling(False)
+        E = dev.e
+        schedule = dev.schedule()
+    n = spec["n"]
+    steps = T - first
+    syn_tick = events / steps
+    fires_tick = float(fpt.mean())
+    # every synapse plastic: the post-synaptic term covers each fire's fan-in (E / neurons on average)
+    step_bytes = SPIKECODE_BYTES_PER_EVENT * syn_tick + STDP_POST_BYTES * fires_tick * E / n
+    ms, launches = kt["fold"]
+    fold_s = ms / max(launches, 1) / 1e3
+    sms, slaunches = kt["scatter"]
+    achieved = step_bytes / fold_s / 1e9
+    return {
+        "metric": "synaptic events/s (spike-code net, 100k neurons, 4 code offsets, STDP on)",
+        "value": events / elapsed,
+        "unit": "synaptic events/s",
+        "steps": steps,
+        "warmup": first,
+        "ms_per_step": elapsed / steps * 1e3,
+        "ticks_per_s": steps / elapsed,
+        "config": {
+            "workload": "100,000 neurons, fan-out 1,000, 80/20 E/I, delays U{1..20}, 4 compartments, 4 code "
+                        "offsets U{0..7} per synapse, STDP on, 1,000 input neurons replaying 10 fixed 8-spike / "
+                        "32-tick patterns (+-1 jitter) over 5 Hz background, threshold 4.6",
+            "neurons": n + spec["n_sources"], "synapses": int(E), "ticks": T,
+            "events_per_tick": syn_tick, "fires_per_tick": fires_tick, "max_fires_per_tick": int(fpt.max()),
+        },
+        "roofline": {"kernel": "k_pfold (partition fold: per-event code patterns and STDP)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": step_bytes, "bytes_per_event": step_bytes / syn_tick,
+                     "mean_launch_ms": fold_s * 1e3, "launches": launches},
+        "scatter_mean_launch_ms": sms / max(slaunches, 1),
+        "schedule": schedule,
+        "generate_seconds": t_gen,
+    }
+
+
 # --------------------------------------------------------------------------- adjoint
 
 def adjoint_form(args, cpu: bool) -> dict:
@@ -504,6 +585,12 @@ def main():
                     vs_baseline=None, dtype="fp32", data="synthetic (seeded generator, built on the device)")
         print(json.dumps(line), flush=True)
         return
+    if args.form == "spikecode":
+        line = spikecode(args)
+        line.update(n_gpus=1, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="fp32",
+                    data="synthetic (seeded generator, built on the device)")
+        print(json.dumps(line), flush=True)
+        return
     if args.form == "adjoint":
         import torch  # noqa: F401  (torch's HIP runtime first; the library then uses it as well)
         line = adjoint_form(args, not args.no_cpu_baseline)
@@ -529,7 +616,7 @@ def main():
                                        "threshold 2.75, same settle/warmup/timed ticks")
         if s.get("kernel_ms_per_tick"):
             line["stdp_on"]["kernel_ms_per_tick"] = s["kernel_ms_per_tick"]
-        line["forms"] = {"batched": batched(args), "adjoint": adjoint_child(args)}
+        line["forms"] = {"batched": batched(args), "adjoint": adjoint_child(args), "spikecode": spikecode(args)}
     if not args.no_cpu_baseline:
         head, others = cpu_baselines(args)
         line["cpu_baseline"] = head

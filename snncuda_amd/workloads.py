@@ -334,11 +334,12 @@ def config1(seed: int = 0, n: int = 100_000, fanout: int = 1000, n_sources: int 
 
 
 def config1_spec(seed: int = 0, n: int = 100_000, fanout: int = 1000, n_sources: int = 1000, source_fanout: int = 1000,
-                 ticks: int = 10_000, threshold: float = 1.1, plastic=True):
+                 ticks: int = 10_000, threshold: float = 1.1, plastic=True, w_exc: float = 0.5, w_inh: float = 1.0):
     """The config-1 network as a device-generator spec plus its pattern-replay commands, without
-    building the synapse arrays on the host (equal to config1(...) built there)."""
+    building the synapse arrays on the host (equal to config1(...) built there). Excitatory weights
+    are U[0, w_exc), inhibitory magnitudes U[0, w_inh)."""
     spec = dict(n=n, n_sources=n_sources, fanout=np.full(n, fanout, np.int64), source_fanout=source_fanout, seed=seed,
-                n_exc=int(round(n * 0.8)), w_exc=0.5, w_inh=1.0, delay_max=20,
+                n_exc=int(round(n * 0.8)), w_exc=w_exc, w_inh=w_inh, delay_max=20,
                 compartments=(SOMA, BASAL, APICAL, INHIBITORY), plastic=PLASTIC_MODES[plastic], threshold=threshold,
                 window=32, sim_thr=0.9, max_refs=1, n_codes=4, code_max=7)
     return spec, pattern_commands(seed, n, n_sources, ticks)

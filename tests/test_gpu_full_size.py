@@ -60,19 +60,21 @@ def test_full_size_results_do_not_depend_on_the_schedule(stdp):
 TICKS1 = 95
 
 
-def _run_digest(spec, injects, schedule):
+def _run_digest(spec, injects, schedule, ticks=TICKS1, track_pre_count=False):
     import hashlib
     with sa.Network(spec=spec, max_events_per_tick=1 << 27, fire_log_capacity=1 << 24, raster_capacity=1 << 24,
-                    track_pre_count=False) as dev:
+                    track_pre_count=track_pre_count) as dev:
         dev.run_commands(injects)
         if schedule is not None:
             dev.set_schedule(partition_shift=schedule[0], scatter_workgroups=schedule[1], graph_ticks=schedule[2],
                              fold_threads=schedule[3])
-        dev.run_until(TICKS1 - 1)
+        dev.run_until(ticks - 1)
         nrn = dev.neurons()
         syn = dev.synapses()
-        d, f = dev.tick_counts(0, TICKS1)
+        d, f = dev.tick_counts(0, ticks)
         out = dict(raster=dev.raster(), delivered=dev.delivered, fired=dev.fired, per_tick=(d, f))
+        if track_pre_count:
+            out["pre_total"] = int(syn["pre_count"].sum())
         out.update({"n_" + k: hashlib.sha1(np.ascontiguousarray(v).tobytes()).hexdigest() for k, v in nrn.items()})
         out.update({"s_" + k: hashlib.sha1(np.ascontiguousarray(v).tobytes()).hexdigest() for k, v in syn.items()})
         return out
@@ -95,6 +97,32 @@ def test_full_size_spike_code_results_do_not_depend_on_the_schedule():
     assert int(d.sum()) == a["delivered"] and int(f.sum()) == a["fired"]
     for sc in ((10, 256, 0, 1024), (11, 512, 8, 256)):
         b = _run_digest(spec, injects, sc)
+        assert a["delivered"] == b["delivered"] and a["fired"] == b["fired"], sc
+        assert np.array_equal(a["raster"], b["raster"]), sc
+        for k in a:
+            if k.startswith(("n_", "s_")):
+                assert a[k] == b[k], (sc, k)
+
+
+def test_full_size_spike_code_whole_run_in_the_pinned_regime():
+    # The spike-code workload's whole 10,000-tick run at the pinned threshold 4.6 (the bench's): the
+    # network holds ~0.5-1 recurrent fires per tick over the ~6 input fires throughout (at 4.5 and
+    # below a fluctuation ignites every neuron within ~1,000 ticks; from 5.0 only the input neurons
+    # fire). Every synaptic event is counted on its synapse, the per-tick counters add up, and every
+    # neuron and synapse field (code patterns, STDP weights) is bit-identical across schedules,
+    # graph replay included.
+    ticks = 10_000
+    spec, cmds = W.config1_spec(0, ticks=ticks, threshold=4.6)
+    injects = [c for c in cmds if c[0] == 0]
+    a = _run_digest(spec, injects, None, ticks=ticks, track_pre_count=True)
+    d, f = a["per_tick"]
+    print(f"  spike-code net, {ticks} ticks: {a['delivered']} delivered, {a['fired']} fired, "
+          f"max {int(f.max())} fires per tick", flush=True)
+    assert 6.0 < f.mean() < 8.0 and f.max() < 100
+    assert int(d.sum()) == a["delivered"] and int(f.sum()) == a["fired"]
+    assert a["pre_total"] == a["delivered"] - len(injects)
+    for sc in ((11, 256, 16, 1024), (12, 512, 0, 256)):
+        b = _run_digest(spec, injects, sc, ticks=ticks, track_pre_count=True)
         assert a["delivered"] == b["delivered"] and a["fired"] == b["fired"], sc
         assert np.array_equal(a["raster"], b["raster"]), sc
         for k in a:
