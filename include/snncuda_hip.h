@@ -201,6 +201,50 @@ int snn_batch_run_until(snn_batch* batch, uint64_t inclusive_tick);
  * totals, arrays of n_nets. delivered needs track_pre_count; either pointer may be NULL. */
 int snn_batch_net_counts(snn_batch* batch, uint64_t* delivered, uint64_t* fired);
 
+/* Non-spiking readouts of a fixed-weight connectome: CudaInferenceSession::run_feedforward_batch and
+ * run_recurrent_feedforward_batch (reference include/snncuda/backends/CudaBackend.h:113-119, kernels
+ * src/backends/cuda/CudaBackend.cu:1260-1528, host sequence :1725-2100). A graph holds the
+ * connectome's out-edges (target, weight, delay max(1, delay_ticks)) and neuron thresholds on the
+ * device; the Connectome's plastic / code / compartment fields are not used. */
+typedef struct snn_graph snn_graph;
+#define SNN_GRAPH_NONE 0xFFFFFFFFu /* a readout / hidden entry that is not a neuron of the connectome */
+int snn_graph_create(const snn_net_desc* desc, int device, snn_graph** out);
+int snn_graph_destroy(snn_graph* graph);
+/* The largest max(1, delay_ticks) over the synapses (1 without synapses). */
+int snn_graph_max_delay(snn_graph* graph, uint32_t* max_delay);
+
+/* Samples are given merged: sample s's inputs are entries [input_ptr[s], input_ptr[s+1]) of
+ * input_neuron / input_weight (neuron indices), one entry per distinct neuron, its weights summed in
+ * sample order, in order of first appearance (the reference merges them the same way,
+ * CudaBackend.cu:1757-1775). readout[r] (and hidden[h]) is a neuron index, or SNN_GRAPH_NONE for an id
+ * that is not a neuron of the connectome (score 0, threshold 1.0, no edges). A neuron listed more
+ * than once maps to its last listing. Sums run in a fixed order: inputs (or sources) in list order,
+ * each source's edges in connectome order.
+ *
+ * Feedforward (:1260-1310): scores[s*R + r] = sum of input_weight * synapse weight over the edges
+ * from sample s's inputs to readout r; spikes[s*R + r] = scores >= the readout's threshold. */
+int snn_graph_feedforward(snn_graph* graph, uint64_t n_samples, const uint64_t* input_ptr,
+                          const uint32_t* input_neuron, const float* input_weight, uint64_t n_readouts,
+                          const uint32_t* readout, float* scores, uint64_t* spikes);
+
+/* Recurrent readout / feedback timeline (:1312-1528, :1851-2100) over time_count =
+ * max_delay * (1 + 2 * iterations) + 1 steps (written to *time_count). top_k_feedback_readouts and
+ * top_k_hidden are the reference options cast to int as its kernels receive them: a readout top-k
+ * <= 0 gates every readout whose step score reaches its threshold; the hidden top-k acts when
+ * 0 < top_k_hidden < n_hidden. Fails with SNN_EINVAL when the timelines
+ * (n_samples * time_count * (n_hidden + n_readouts) floats) exceed a quarter of free device memory. */
+typedef struct snn_recurrent_opts {
+    uint32_t iterations;
+    float feedback_decay;
+    int32_t top_k_feedback_readouts;
+    int32_t top_k_hidden;
+    int32_t use_scores_not_spikes;
+} snn_recurrent_opts;
+int snn_graph_recurrent(snn_graph* graph, uint64_t n_samples, const uint64_t* input_ptr,
+                        const uint32_t* input_neuron, const float* input_weight, uint64_t n_readouts,
+                        const uint32_t* readout, uint64_t n_hidden, const uint32_t* hidden,
+                        const snn_recurrent_opts* opts, float* scores, uint64_t* spikes, uint64_t* time_count);
+
 /* Recording of a run for the fixed-spike adjoint (no counterpart in the reference). Switched on
  * before the first tick, the step keeps, for every fire of the raster, the index of the event the
  * neuron fired on among its ordered events of that tick, and keeps every delivered inject (neuron,

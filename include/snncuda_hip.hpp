@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -393,6 +394,91 @@ public:
 
     void reset() { state_ = Network(); }
 
+    // CudaInferenceSession::run_feedforward_batch (reference src/backends/cuda/CudaBackend.cu:1725-1849)
+    // on snn_graph_feedforward: inputs merged per neuron in order of first appearance (unknown ids
+    // skipped), delivered = merged inputs, scores and 0/1 spike counts in readout order, fired = the
+    // spike counts' sum.
+    BatchResult feedforward(const std::vector<Sample>& samples, const Options& options) {
+        const auto start = std::chrono::steady_clock::now();
+        BatchResult batch{};
+        batch.executed = true;
+        batch.samples.resize(samples.size());
+        if (samples.empty() || options.readout_neurons.empty()) return batch;
+        Merged m = merge(samples);
+        const std::vector<std::uint32_t> readout = dense_list(options.readout_neurons);
+        const std::size_t S = samples.size(), R = readout.size();
+        std::vector<float> scores(S * R);
+        std::vector<std::uint64_t> spikes(S * R);
+        check(snn_graph_feedforward(graph(), S, m.ptr.data(), m.neuron.data(), m.weight.data(), R, readout.data(),
+                                    scores.data(), spikes.data()),
+              "snn_graph_feedforward");
+        fill_readouts(batch, m, scores, spikes, R, 0);
+        batch.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        return batch;
+    }
+
+    // CudaInferenceSession::run_recurrent_feedforward_batch (:1851-2100) on snn_graph_recurrent. With
+    // use_full_spike_timing the samples run through the spiking step (run) for max_timing_steps, or
+    // max_delay * (1 + 2 * iterations) + 1 ticks when that is 0. Hidden neurons default to every
+    // neuron that is not a readout, in connectome order (the reference takes them in the iteration
+    // order of its id hash map, :2237-2251).
+    template <class RecurrentOptions>
+    BatchResult recurrent(const std::vector<Sample>& samples, const RecurrentOptions& options) {
+        if (options.use_full_spike_timing) {
+            Options timing{};
+            timing.max_steps = options.max_timing_steps == 0
+                                   ? static_cast<std::uint32_t>(static_cast<std::size_t>(max_delay()) *
+                                                                    (1u + 2u * static_cast<std::size_t>(options.iterations)) +
+                                                                1u)
+                                   : options.max_timing_steps;
+            timing.reset_state_between_samples = true;
+            timing.readout_neurons = options.readout_neurons;
+            return run(samples, timing);
+        }
+        const auto start = std::chrono::steady_clock::now();
+        BatchResult batch{};
+        batch.executed = true;
+        batch.samples.resize(samples.size());
+        if (samples.empty() || options.readout_neurons.empty()) return batch;
+        std::vector<std::uint32_t> hidden;
+        if (!options.hidden_neurons.empty()) {
+            hidden = dense_list(options.hidden_neurons);
+        } else {
+            std::unordered_set<std::uint64_t> ro;
+            for (const auto id : options.readout_neurons) ro.insert(static_cast<std::uint64_t>(id));
+            std::vector<std::uint64_t> id_of(p_.threshold.size());
+            for (const auto& [id, x] : p_.index) id_of[x] = id;
+            for (std::uint32_t x = 0; x < id_of.size(); ++x)
+                if (!ro.count(id_of[x])) hidden.push_back(x);
+        }
+        if (hidden.empty()) return batch;
+        Merged m = merge(samples);
+        const std::vector<std::uint32_t> readout = dense_list(options.readout_neurons);
+        const std::size_t S = samples.size(), R = readout.size();
+        std::vector<float> scores(S * R);
+        std::vector<std::uint64_t> spikes(S * R);
+        snn_recurrent_opts o{};
+        o.iterations = options.iterations;
+        o.feedback_decay = options.feedback_decay;
+        o.top_k_feedback_readouts = static_cast<std::int32_t>(options.top_k_feedback_readouts);
+        o.top_k_hidden = static_cast<std::int32_t>(options.top_k_hidden);
+        o.use_scores_not_spikes = options.use_scores_not_spikes ? 1 : 0;
+        std::uint64_t ticks = 0;
+        check(snn_graph_recurrent(graph(), S, m.ptr.data(), m.neuron.data(), m.weight.data(), R, readout.data(),
+                                  hidden.size(), hidden.data(), &o, scores.data(), spikes.data(), &ticks),
+              "snn_graph_recurrent");
+        fill_readouts(batch, m, scores, spikes, R, ticks);
+        batch.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        return batch;
+    }
+
+    // the largest max(1, delay_ticks) over the connectome's synapses
+    std::uint32_t max_delay() {
+        std::uint32_t d = 0;
+        check(snn_graph_max_delay(graph(), &d), "snn_graph_max_delay");
+        return d;
+    }
+
 private:
     template <class Result>
     void fill_debug(Result& r, const Options& options, std::uint64_t synaptic, std::uint64_t fired) const {
@@ -495,9 +581,77 @@ private:
         }
     }
 
+    struct Merged {
+        std::vector<std::uint64_t> ptr{0};
+        std::vector<std::uint32_t> neuron;
+        std::vector<float> weight;
+    };
+
+    // per sample: one entry per distinct known neuron, weights summed in input order
+    Merged merge(const std::vector<Sample>& samples) const {
+        Merged m;
+        std::unordered_map<std::uint32_t, std::size_t> at;
+        for (const auto& sample : samples) {
+            at.clear();
+            for (const auto& in : sample.inputs) {
+                const auto it = p_.index.find(static_cast<std::uint64_t>(in.neuron));
+                if (it == p_.index.end()) continue;
+                const auto [pos, fresh] = at.emplace(it->second, m.neuron.size());
+                if (fresh) {
+                    m.neuron.push_back(it->second);
+                    m.weight.push_back(in.weight);
+                } else {
+                    m.weight[pos->second] += in.weight;
+                }
+            }
+            m.ptr.push_back(m.neuron.size());
+        }
+        return m;
+    }
+
+    template <class Ids>
+    std::vector<std::uint32_t> dense_list(const Ids& ids) const {
+        std::vector<std::uint32_t> out;
+        out.reserve(ids.size());
+        for (const auto id : ids) {
+            const auto it = p_.index.find(static_cast<std::uint64_t>(id));
+            out.push_back(it == p_.index.end() ? SNN_GRAPH_NONE : it->second);
+        }
+        return out;
+    }
+
+    void fill_readouts(BatchResult& batch, const Merged& m, const std::vector<float>& scores,
+                       const std::vector<std::uint64_t>& spikes, std::size_t R, std::uint64_t ticks) const {
+        for (std::size_t s = 0; s < batch.samples.size(); ++s) {
+            auto& r = batch.samples[s];
+            r.delivered_spikes = m.ptr[s + 1] - m.ptr[s];
+            r.readout_scores.assign(scores.begin() + s * R, scores.begin() + (s + 1) * R);
+            r.readout_spike_counts.assign(spikes.begin() + s * R, spikes.begin() + (s + 1) * R);
+            for (const auto c : r.readout_spike_counts) r.fired_spikes += c;
+            r.debug.ticks_processed = ticks;
+            r.debug.processed_events = r.delivered_spikes;
+            r.debug.dendritic_integrations = r.delivered_spikes;
+        }
+    }
+
+    snn_graph* graph() {
+        if (!graph_) {
+            const snn_net_desc d = p_.desc();
+            snn_graph* g = nullptr;
+            check(snn_graph_create(&d, 0, &g), "snn_graph_create");
+            graph_.reset(g);
+        }
+        return graph_.get();
+    }
+
+    struct GraphFree {
+        void operator()(snn_graph* g) const noexcept { snn_graph_destroy(g); }
+    };
+
     using Counts = Network::Counts;
     PackedConnectome p_;
     Network state_;
+    std::unique_ptr<snn_graph, GraphFree> graph_;
 };
 
 }  // namespace snncuda_hip

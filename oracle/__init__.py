@@ -66,6 +66,12 @@ def lib() -> ctypes.CDLL:
         L.adjo_loss_target.restype = ctypes.c_double
         L.adjo_grad.argtypes = [P] * 8
         L.adjo_grad.restype = ctypes.c_double
+        U64 = ctypes.c_uint64
+        L.roo_feedforward.argtypes = [U64, P, U64, P, P, P, U64, P, P, P, U64, P, P, P]
+        L.roo_recurrent.argtypes = ([U64, P, U64, P, P, P, P, U64, P, P, P, U64, P, U64, P, ctypes.c_uint32,
+                                     ctypes.c_float, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P, P, P])
+        L.roo_feedforward.restype = ctypes.c_int
+        L.roo_recurrent.restype = ctypes.c_int
         L.snno_run_file.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
         L.snno_run_file.restype = ctypes.c_int
         _lib = L
@@ -320,3 +326,50 @@ class AdjointOracle:
         gt = np.zeros(4, np.float64)
         loss = lib().adjo_grad(self.h, _p(w), _p(alpha), _p(tau), _p(target), _p(gw), _p(ga), _p(gt))
         return float(loss), gw, ga, gt
+
+
+# --------------------------------------------------------------------------- readouts
+
+def _graph_arrays(net):
+    return (np.ascontiguousarray(net.threshold, np.float32), np.ascontiguousarray(net.src, np.uint32),
+            np.ascontiguousarray(net.tgt, np.uint32), np.ascontiguousarray(net.weight, np.float32),
+            np.ascontiguousarray(net.delay, np.uint32))
+
+
+def readout_feedforward(net, ptr, neuron, weight, readout):
+    """oracle/readout_oracle.c's feedforward readout: (scores float32 [S, R], spikes uint64 [S, R])."""
+    thr, src, tgt, w, _ = _graph_arrays(net)
+    ptr = np.ascontiguousarray(ptr, np.uint64)
+    neuron = np.ascontiguousarray(neuron, np.uint32)
+    weight = np.ascontiguousarray(weight, np.float32)
+    readout = np.ascontiguousarray(readout, np.uint32)
+    S, R = len(ptr) - 1, len(readout)
+    scores = np.zeros((S, R), np.float32)
+    spikes = np.zeros((S, R), np.uint64)
+    rc = lib().roo_feedforward(len(thr), _p(thr), len(src), _p(src), _p(tgt), _p(w), S, _p(ptr), _p(neuron),
+                               _p(weight), R, _p(readout), _p(scores), _p(spikes))
+    if rc:
+        raise MemoryError("roo_feedforward failed")
+    return scores, spikes
+
+
+def readout_recurrent(net, ptr, neuron, weight, readout, hidden, *, iterations=1, feedback_decay=0.5,
+                      top_k_feedback_readouts=1, top_k_hidden=0, use_scores_not_spikes=True):
+    """oracle/readout_oracle.c's recurrent readout: (scores, spikes, time_count)."""
+    thr, src, tgt, w, dl = _graph_arrays(net)
+    ptr = np.ascontiguousarray(ptr, np.uint64)
+    neuron = np.ascontiguousarray(neuron, np.uint32)
+    weight = np.ascontiguousarray(weight, np.float32)
+    readout = np.ascontiguousarray(readout, np.uint32)
+    hidden = np.ascontiguousarray(hidden, np.uint32)
+    S, R = len(ptr) - 1, len(readout)
+    scores = np.zeros((S, R), np.float32)
+    spikes = np.zeros((S, R), np.uint64)
+    T = np.zeros(1, np.uint64)
+    rc = lib().roo_recurrent(len(thr), _p(thr), len(src), _p(src), _p(tgt), _p(w), _p(dl), S, _p(ptr), _p(neuron),
+                             _p(weight), R, _p(readout), len(hidden), _p(hidden), iterations, feedback_decay,
+                             top_k_feedback_readouts, top_k_hidden, int(use_scores_not_spikes), _p(scores),
+                             _p(spikes), _p(T))
+    if rc:
+        raise MemoryError("roo_recurrent failed")
+    return scores, spikes, int(T[0])

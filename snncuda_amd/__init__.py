@@ -73,6 +73,15 @@ class Schedule(ctypes.Structure):
                 ("graph_ticks", ctypes.c_uint32), ("fold_threads", ctypes.c_uint32)]
 
 
+class RecurrentOptions(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint32), ("feedback_decay", ctypes.c_float),
+                ("top_k_feedback_readouts", ctypes.c_int32), ("top_k_hidden", ctypes.c_int32),
+                ("use_scores_not_spikes", ctypes.c_int32)]
+
+
+GRAPH_NONE = 0xFFFFFFFF  # a readout / hidden entry that is not a neuron of the connectome
+
+
 def library_path() -> pathlib.Path:
     return pathlib.Path(os.environ.get("SNNCUDA_HIP_LIBRARY", PKG / LIB_NAME))
 
@@ -125,6 +134,14 @@ def _load() -> ctypes.CDLL:
     L.snn_batch_net_counts.argtypes = [P, P, P]
     L.snn_net_set_profiling.argtypes = [P, I]
     L.snn_net_kernel_times.argtypes = [P, P, P, I, ctypes.c_char_p, U64]
+    L.snn_graph_create.argtypes = [ctypes.POINTER(_Desc), I, ctypes.POINTER(P)]
+    L.snn_graph_destroy.argtypes = [P]
+    L.snn_graph_max_delay.argtypes = [P, P]
+    L.snn_graph_feedforward.argtypes = [P, U64, P, P, P, U64, P, P, P]
+    L.snn_graph_recurrent.argtypes = [P, U64, P, P, P, U64, P, U64, P, ctypes.POINTER(RecurrentOptions), P, P, P]
+    for name in ("snn_graph_create", "snn_graph_destroy", "snn_graph_max_delay", "snn_graph_feedforward",
+                 "snn_graph_recurrent"):
+        getattr(L, name).restype = I
     for name in ("snn_net_create", "snn_net_create_ei", "snn_net_destroy", "snn_net_inject", "snn_net_run_until", "snn_net_run_until_static", "snn_net_counts",
                  "snn_net_read_tick_counts", "snn_net_read_raster", "snn_net_read_neurons",
                  "snn_net_read_synapses", "snn_net_read_plasticity", "snn_net_set_profiling",
@@ -412,6 +429,94 @@ def _desc(net):
         (net.receptor, np.uint8), (net.plastic, np.uint8), (net.code_ptr, np.uint64), (net.codes, np.uint32))]
     d = _Desc(int(a[0].shape[0]), _p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), int(a[4].shape[0]), *[_p(x) for x in a[4:]])
     return d, a
+
+
+def merge_inputs(samples, n_neurons: int):
+    """Samples of (neuron, weight) inputs merged as the session merges them: per sample one entry per
+    distinct neuron in order of first appearance, its weights summed in input order (float32); neurons
+    outside [0, n_neurons) are skipped. Returns (ptr uint64[S+1], neuron uint32, weight float32)."""
+    ptr, xs, ws = [0], [], []
+    for sample in samples:
+        at = {}
+        for x, w in sample:
+            x = int(x)
+            if x < 0 or x >= n_neurons:
+                continue
+            if x in at:
+                ws[at[x]] = np.float32(ws[at[x]] + np.float32(w))
+            else:
+                at[x] = len(xs)
+                xs.append(x)
+                ws.append(np.float32(w))
+        ptr.append(len(xs))
+    return (np.asarray(ptr, np.uint64), np.asarray(xs, np.uint32).reshape(-1),
+            np.asarray(ws, np.float32).reshape(-1))
+
+
+class Graph:
+    """A connectome's fixed-weight edges on the device for the non-spiking readouts of
+    ``CudaInferenceSession`` (snn_graph_*; reference ``run_feedforward_batch`` /
+    ``run_recurrent_feedforward_batch``, ``include/snncuda/backends/CudaBackend.h:113-119``)."""
+
+    def __init__(self, net, device: int = 0):
+        d, keep = _desc(net)
+        self.n, self.e = int(d.n_neurons), int(d.n_synapses)
+        h = ctypes.c_void_p()
+        _check(lib().snn_graph_create(ctypes.byref(d), device, ctypes.byref(h)))
+        self._h = h
+        del keep
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().snn_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def max_delay(self) -> int:
+        v = ctypes.c_uint32()
+        _check(lib().snn_graph_max_delay(self._h, ctypes.byref(v)))
+        return int(v.value)
+
+    def feedforward(self, ptr, neuron, weight, readout):
+        """Merged inputs (``merge_inputs``) and readout neuron indices (``GRAPH_NONE`` for an unknown id)
+        -> (scores float32 [S, R], spikes uint64 [S, R])."""
+        ptr = np.ascontiguousarray(ptr, np.uint64)
+        neuron = np.ascontiguousarray(neuron, np.uint32)
+        weight = np.ascontiguousarray(weight, np.float32)
+        readout = np.ascontiguousarray(readout, np.uint32)
+        S, R = len(ptr) - 1, len(readout)
+        scores = np.zeros((S, R), np.float32)
+        spikes = np.zeros((S, R), np.uint64)
+        _check(lib().snn_graph_feedforward(self._h, S, _p(ptr), _p(neuron), _p(weight), R, _p(readout),
+                                           _p(scores), _p(spikes)))
+        return scores, spikes
+
+    def recurrent(self, ptr, neuron, weight, readout, hidden, *, iterations: int = 1, feedback_decay: float = 0.5,
+                  top_k_feedback_readouts: int = 1, top_k_hidden: int = 0, use_scores_not_spikes: bool = True):
+        """-> (scores float32 [S, R], spikes uint64 [S, R], time_count)."""
+        ptr = np.ascontiguousarray(ptr, np.uint64)
+        neuron = np.ascontiguousarray(neuron, np.uint32)
+        weight = np.ascontiguousarray(weight, np.float32)
+        readout = np.ascontiguousarray(readout, np.uint32)
+        hidden = np.ascontiguousarray(hidden, np.uint32)
+        S, R = len(ptr) - 1, len(readout)
+        scores = np.zeros((S, R), np.float32)
+        spikes = np.zeros((S, R), np.uint64)
+        ticks = ctypes.c_uint64()
+        o = RecurrentOptions(iterations, feedback_decay, top_k_feedback_readouts, top_k_hidden,
+                             int(use_scores_not_spikes))
+        _check(lib().snn_graph_recurrent(self._h, S, _p(ptr), _p(neuron), _p(weight), R, _p(readout), len(hidden),
+                                         _p(hidden), ctypes.byref(o), _p(scores), _p(spikes), ctypes.byref(ticks)))
+        return scores, spikes, int(ticks.value)
 
 
 class Batch:

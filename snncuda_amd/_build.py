@@ -12,7 +12,7 @@ import subprocess
 
 PKG = pathlib.Path(__file__).resolve().parent
 ROOT = PKG.parent
-SOURCES = [PKG / "csrc" / "snn_step.hip", PKG / "csrc" / "snn_adjoint.hip"]
+SOURCES = [PKG / "csrc" / "snn_step.hip", PKG / "csrc" / "snn_adjoint.hip", PKG / "csrc" / "snn_readout.hip"]
 HEADERS = sorted((PKG / "csrc").glob("*.h")) + [ROOT / "include" / "snncuda_hip.h"]
 OUT = PKG / "libsnncuda_hip.so"
 

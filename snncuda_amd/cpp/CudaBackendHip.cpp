@@ -5,9 +5,9 @@
 // Built inside the reference tree in place of src/backends/cuda/CudaBackend.cu, with include/ of
 // this repository on the include path and libsnncuda_hip.so linked. The class declarations,
 // result types and callers stay unchanged. Errors raised by the device library surface as
-// snncuda_hip::Error, a std::runtime_error, as the reference's check_cuda does. The non-spiking
-// readouts (run_feedforward_batch, run_recurrent_feedforward_batch, CudaBackend.cu:1260-1528) are
-// not on this backend and throw.
+// snncuda_hip::Error, a std::runtime_error, as the reference's check_cuda does. The spiking
+// session runs on the step (snn_batch_*), the non-spiking readouts (run_feedforward_batch,
+// run_recurrent_feedforward_batch, CudaBackend.cu:1260-1528) on snn_graph_*.
 
 #include "snncuda/backends/CudaBackend.h"
 #include "snncuda_hip.hpp"
@@ -47,15 +47,21 @@ CudaInferenceBatchResult CudaInferenceSession::run_batch(
 }
 
 CudaInferenceBatchResult CudaInferenceSession::run_feedforward_batch(
-    const std::vector<CudaInferenceSample>&,
-    const CudaInferenceOptions&) {
-    throw std::runtime_error("run_feedforward_batch is not provided by the HIP backend");
+    const std::vector<CudaInferenceSample>& samples,
+    const CudaInferenceOptions& options) {
+    if (!impl_->available_) {
+        return {};
+    }
+    return impl_->runner_.feedforward(samples, options);
 }
 
 CudaInferenceBatchResult CudaInferenceSession::run_recurrent_feedforward_batch(
-    const std::vector<CudaInferenceSample>&,
-    const CudaRecurrentFeedforwardOptions&) {
-    throw std::runtime_error("run_recurrent_feedforward_batch is not provided by the HIP backend");
+    const std::vector<CudaInferenceSample>& samples,
+    const CudaRecurrentFeedforwardOptions& options) {
+    if (!impl_->available_) {
+        return {};
+    }
+    return impl_->runner_.recurrent(samples, options);
 }
 
 void CudaInferenceSession::reset_state() {

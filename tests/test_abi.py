@@ -53,3 +53,12 @@ def test_batch_create_without_device_fails_cleanly():
     with pytest.raises(sa.SNNError) as e:
         sa.Batch([W.feature_net(0), W.feature_net(1)])
     assert e.value.code == sa.SNN_ENODEVICE
+
+
+def test_graph_create_without_device_fails_cleanly():
+    from snncuda_amd import workloads as W
+    if sa.device_count() != 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(sa.SNNError) as e:
+        sa.Graph(W.feature_net(0))
+    assert e.value.code == sa.SNN_ENODEVICE

@@ -110,3 +110,19 @@ def test_inference_session_run_batch_matches_reference_cpu(seed, tmp_path):
     oracle.write_net(path, W.feature_net(seed))
     r = _run(["session", str(path), "80", "5"])
     assert r.returncode == 0, r.stderr
+
+
+SESSION_TESTS = ROOT / "oracle" / "_ref" / "cuda_synapse_dendrite_tests"
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not SESSION_TESTS.exists(), reason="oracle/_ref session tests not built")
+def test_reference_cuda_synapse_dendrite_tests_pass():
+    # the reference's tests/cuda_synapse_dendrite_tests.cpp, unchanged, on the drop-in: LTP / LTD /
+    # inhibitory / coded synapses against its CPU runtime, the inference session, the feedforward and
+    # recurrent readouts (:306-421) and the persistent-session timing check
+    print("reference tests/cuda_synapse_dendrite_tests.cpp on the drop-in backend", flush=True)
+    r = subprocess.run([str(SESSION_TESTS)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "All CUDA synapse/dendrite tests passed" in r.stdout
