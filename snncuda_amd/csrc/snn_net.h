@@ -22,6 +22,7 @@
 #include <atomic>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -215,12 +216,20 @@ struct PackError {
     }
 };
 
+// Points of the pack after which arrays of the HostNet are final, so their upload can start while
+// the pack goes on:
+//   Slots:  params, slot_of, tgt, w, meta, delay, expected (and n, e, nve, lmax, ve_identity,
+//           any_plastic, expected_all)
+//   Groups: row, grp, ve
+//   Stdp:   tpos, in_ptr, stdp, tpos_conn (and code_compact)
+enum class PackStage { Slots, Groups, Stdp };
+using PackHook = std::function<void(PackStage, const HostNet&)>;
+
 // Throws std::invalid_argument with a message for inputs outside the device step's limits (the
 // message of the first offending neuron or synapse in connectome order). The passes over synapses
 // run on `threads` threads (0: pack_threads(), one below 65,536 synapses); every output equals a
-// sequential pass's.
-inline HostNet build_host_net(const NetDesc& d, unsigned threads = 0) {
-    HostNet h;
+// sequential pass's. hook (optional) is called at each PackStage.
+inline void build_host_net_into(HostNet& h, const NetDesc& d, unsigned threads = 0, const PackHook& hook = {}) {
     if (d.n_neurons == 0) throw std::invalid_argument("network has no neurons");
     if (d.n_neurons >= 0xFFFFFFFFull) throw std::invalid_argument("more than 2^32-2 neurons");
     if (d.n_synapses >= 0xFFFFFFFFull) throw std::invalid_argument("more than 2^32-2 synapses on one device");
@@ -360,6 +369,7 @@ inline HostNet build_host_net(const NetDesc& d, unsigned threads = 0) {
     for (const PackError& q : perr) err.merge(q);
     err.raise();
     pvec<uint32_t>().swap(order);
+    if (hook) hook(PackStage::Slots, h);
 
     // virtual edges and group table
     h.row.assign((uint64_t)n + 1, 0);
@@ -426,6 +436,7 @@ inline HostNet build_host_net(const NetDesc& d, unsigned threads = 0) {
     }
     pvec<uint64_t>().swap(cnt);
     pvec<uint64_t>().swap(vc);
+    if (hook) hook(PackStage::Groups, h);
 
     // compact code record only when every target neuron keeps at most one learned pattern
     std::vector<uint8_t> one(T, 1);
@@ -474,6 +485,12 @@ inline HostNet build_host_net(const NetDesc& d, unsigned threads = 0) {
             }
         });
     }
+    if (hook) hook(PackStage::Stdp, h);
+}
+
+inline HostNet build_host_net(const NetDesc& d, unsigned threads = 0) {
+    HostNet h;
+    build_host_net_into(h, d, threads);
     return h;
 }
 

@@ -21,10 +21,14 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -1445,6 +1449,128 @@ T* dev_upload(std::vector<DevBuf>& owned, const std::vector<T, A>& v) {
     return p;
 }
 
+// Upload of the packed arrays while the host pack goes on. Each array is queued as soon as the pack
+// has finished it (PackStage); a worker thread moves it chunk by chunk through a ring of pinned
+// staging buffers: a parallel memcpy into a free buffer, then an async copy from it on the
+// uploader's own stream, so the DMA of the slot arrays runs during the group-table and STDP passes.
+// A network below kStreamMinSynapses copies directly with hipMemcpy (the pinned ring would cost more
+// than it saves). The queued host arrays must stay alive until finish() (or the destructor) returns.
+constexpr uint64_t kStreamMinSynapses = 1ull << 22;
+class Uploader {
+public:
+    Uploader(int device, bool streamed) : device_(device), streamed_(streamed) {
+        if (!streamed_) return;
+        check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+        for (int k = 0; k < kRing; ++k) {
+            check(hipHostMalloc(&ring_[k], kChunk, hipHostMallocDefault), "hipHostMalloc");
+            check(hipEventCreateWithFlags(&ev_[k], hipEventDisableTiming), "hipEventCreate");
+        }
+        worker_ = std::thread([this] { run(); });
+    }
+    ~Uploader() {
+        try {
+            finish();
+        } catch (...) {
+        }
+    }
+    Uploader(const Uploader&) = delete;
+    Uploader& operator=(const Uploader&) = delete;
+
+    template <typename T, typename A>
+    T* upload(std::vector<DevBuf>& owned, const std::vector<T, A>& v) {
+        if (!streamed_) return dev_upload(owned, v);
+        T* p = dev_alloc<T>(owned, v.size(), false);
+        if (!v.empty()) {
+            std::lock_guard<std::mutex> lk(m_);
+            q_.push_back(Job{(char*)p, (const char*)v.data(), v.size() * sizeof(T)});
+            cv_.notify_one();
+        }
+        return p;
+    }
+
+    // every queued copy done; rethrows the worker's error
+    void finish() {
+        if (!streamed_) return;
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            closing_ = true;
+            cv_.notify_one();
+        }
+        if (worker_.joinable()) worker_.join();
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        for (int k = 0; k < kRing; ++k) {
+            if (ev_[k]) (void)hipEventDestroy(ev_[k]);
+            if (ring_[k]) (void)hipHostFree(ring_[k]);
+            ev_[k] = nullptr;
+            ring_[k] = nullptr;
+        }
+        if (stream_) (void)hipStreamDestroy(stream_);
+        stream_ = nullptr;
+        streamed_ = false;
+        if (!error_.empty()) throw HipError(error_);
+    }
+
+private:
+    static constexpr size_t kChunk = 64ull << 20;
+    static constexpr int kRing = 3;
+    static constexpr unsigned kCopyThreads = 4;
+    struct Job {
+        char* dst;
+        const char* src;
+        size_t bytes;
+    };
+
+    void run() {
+        try {
+            check(hipSetDevice(device_), "hipSetDevice");
+            uint64_t k = 0;
+            while (true) {
+                Job j;
+                {
+                    std::unique_lock<std::mutex> lk(m_);
+                    cv_.wait(lk, [&] { return closing_ || !q_.empty(); });
+                    if (q_.empty()) return;
+                    j = q_.front();
+                    q_.pop_front();
+                }
+                for (size_t off = 0; off < j.bytes; off += kChunk, ++k) {
+                    const size_t len = std::min(kChunk, j.bytes - off);
+                    const int slot = (int)(k % kRing);
+                    check(hipEventSynchronize(ev_[slot]), "hipEventSynchronize");  // the buffer's last copy
+                    char* buf = (char*)ring_[slot];
+                    const char* src = j.src + off;
+                    std::thread part[kCopyThreads - 1];
+                    const size_t per = (len + kCopyThreads - 1) / kCopyThreads;
+                    for (unsigned t = 1; t < kCopyThreads; ++t) {
+                        const size_t a = std::min(len, t * per), b = std::min(len, (t + 1) * per);
+                        part[t - 1] = std::thread([=] { if (b > a) std::memcpy(buf + a, src + a, b - a); });
+                    }
+                    std::memcpy(buf, src, std::min(len, per));
+                    for (auto& th : part) th.join();
+                    check(hipMemcpyAsync(j.dst + off, buf, len, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync H2D");
+                    check(hipEventRecord(ev_[slot], stream_), "hipEventRecord");
+                }
+            }
+        } catch (const std::exception& e) {
+            std::lock_guard<std::mutex> lk(m_);
+            error_ = e.what();
+            q_.clear();
+        }
+    }
+
+    int device_;
+    bool streamed_;
+    hipStream_t stream_ = nullptr;
+    void* ring_[kRing] = {};
+    hipEvent_t ev_[kRing] = {};
+    std::thread worker_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<Job> q_;
+    bool closing_ = false;
+    std::string error_;
+};
+
 struct PendingInject {
     uint32_t neuron;
     uint64_t tick;
@@ -2138,31 +2264,63 @@ std::unique_ptr<snn_net> create_net(const NetDesc& d, const snn_options& o) {
                          !d.receptor || !d.plastic))
         throw std::invalid_argument("null synapse array");
     (void)default_partition_shift(d.n_neurons);  // a network too large for one device fails here
-    HostNet h = build_host_net(d);
 
+    // net before h before up: on any error the uploader drains first, then the host arrays it reads
+    // go, then the device buffers
     std::unique_ptr<snn_net> net(new snn_net());
-    open_net(net.get(), o, h.n, h.e, h.lmax, h.any_plastic, h.ve_identity, h.code_compact);
+    net->device = o.device;
+    check(hipSetDevice(o.device), "hipSetDevice");
     auto& own = net->owned;
     Step& S = net->S;
-    S.row = dev_upload(own, h.row);
-    S.grp = dev_upload(own, h.grp);
-    S.ve = h.ve_identity ? nullptr : dev_upload(own, h.ve);
-    S.tgt = dev_upload(own, h.tgt);
-    S.w = dev_upload(own, h.w);
-    S.meta = dev_upload(own, h.meta);
-    S.delay = dev_upload(own, h.delay);
-    S.expected = h.expected.empty() ? nullptr : dev_upload(own, h.expected);
-    S.expected_all = h.expected_all ? h.expected_all : 2;
-    if (h.any_plastic) {
-        S.tpos = dev_upload(own, h.tpos);
-        S.in_ptr = dev_upload(own, h.in_ptr);
-        S.stdp = dev_upload(own, h.stdp);
-        net->tpos_conn_dev = dev_upload(own, h.tpos_conn);
-        net->n_tpos = h.stdp.size();
+    HostNet h;
+    // SNNCUDA_CREATE_TIMING=1 prints the phases of the call to stderr
+    const bool timing = std::getenv("SNNCUDA_CREATE_TIMING") != nullptr;
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto t_pack = t0, t_up = t0;
+    {
+        Uploader up(o.device, d.n_synapses >= kStreamMinSynapses);
+        build_host_net_into(h, d, 0, [&](PackStage stage, const HostNet& hh) {
+            switch (stage) {
+            case PackStage::Slots:
+                S.tgt = up.upload(own, hh.tgt);
+                S.w = up.upload(own, hh.w);
+                S.meta = up.upload(own, hh.meta);
+                S.delay = up.upload(own, hh.delay);
+                S.expected = hh.expected.empty() ? nullptr : up.upload(own, hh.expected);
+                S.np = up.upload(own, hh.params);
+                net->slot_of = up.upload(own, hh.slot_of);
+                break;
+            case PackStage::Groups:
+                S.row = up.upload(own, hh.row);
+                S.grp = up.upload(own, hh.grp);
+                S.ve = hh.ve_identity ? nullptr : up.upload(own, hh.ve);
+                break;
+            case PackStage::Stdp:
+                if (hh.any_plastic) {
+                    S.tpos = up.upload(own, hh.tpos);
+                    S.in_ptr = up.upload(own, hh.in_ptr);
+                    S.stdp = up.upload(own, hh.stdp);
+                    net->tpos_conn_dev = up.upload(own, hh.tpos_conn);
+                    net->n_tpos = hh.stdp.size();
+                }
+                break;
+            }
+        });
+        t_pack = clk::now();
+        up.finish();
+        t_up = clk::now();
     }
-    S.np = dev_upload(own, h.params);
-    net->slot_of = dev_upload(own, h.slot_of);
+    S.expected_all = h.expected_all ? h.expected_all : 2;
+    open_net(net.get(), o, h.n, h.e, h.lmax, h.any_plastic, h.ve_identity, h.code_compact);
     finish_net(net.get(), o, h.nve, h.params);
+    if (timing) {
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        auto sec = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+        std::fprintf(stderr, "snn_net_create: %llu synapses, pack %.3f s, upload after the pack %.3f s, device state %.3f s (%s upload)\n",
+                     (unsigned long long)d.n_synapses, sec(t0, t_pack), sec(t_pack, t_up), sec(t_up, clk::now()),
+                     d.n_synapses >= kStreamMinSynapses ? "streamed" : "direct");
+    }
     return net;
 }
 
