@@ -67,9 +67,15 @@ typedef struct snn_options {
     int32_t track_pre_count;        /* keep per-synapse delivered counts (pre_spike_count) */
     int32_t device;                 /* HIP device ordinal */
     uint64_t learned_pattern_slots; /* learned temporal patterns held per neuron (0: the network's
-                                       largest max_reference_patterns, as many as a quarter of the
-                                       device's memory holds); a neuron whose max_reference_patterns
-                                       is larger fails the run once it holds that many */
+                                       largest max_reference_patterns); a neuron whose
+                                       max_reference_patterns is larger fails the run once it holds
+                                       that many */
+    uint64_t pattern_device_pages;  /* the learned-pattern pool in pages of 1,024 neurons: 0 keeps it
+                                       on the device when a quarter of device memory holds it, else
+                                       in pinned host memory behind a device cache of as many pages
+                                       as that quarter holds; n > 0 gives a device cache of n pages
+                                       (the whole pool on the device when n covers the network).
+                                       Results do not depend on where the pages are. */
 } snn_options;
 
 typedef struct snn_net snn_net;
@@ -125,6 +131,12 @@ int snn_net_run_until(snn_net* net, uint64_t inclusive_tick);
  * pending inject) fails with SNN_EINVAL and changes nothing. All calls on one device share its
  * array: calls that overlap on one device race. */
 int snn_net_run_until_static(snn_net* net, uint64_t inclusive_tick);
+
+/* No counterpart in the reference (its NeuronStateCache pages neuron state on the CPU,
+ * src/runtime/NeuronStateCache.cpp:5-43): the learned-pattern pool's pages (of 1,024 neurons), how
+ * many of them the device cache holds (0: the whole pool is on the device) and the page copies made
+ * so far. Any pointer may be NULL. */
+int snn_net_pattern_paging(snn_net* net, uint64_t* pages, uint64_t* device_pages, uint64_t* moves);
 
 /* delivered_spike_count / fired_spike_count (NetworkPropagator.h:27-28) and the next tick. */
 int snn_net_counts(snn_net* net, uint64_t* delivered, uint64_t* fired, uint64_t* next_tick);

@@ -55,7 +55,8 @@ class _Desc(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("max_events_per_tick", ctypes.c_uint64), ("fire_log_capacity", ctypes.c_uint64),
                 ("raster_capacity", ctypes.c_uint64), ("track_pre_count", ctypes.c_int32),
-                ("device", ctypes.c_int32), ("learned_pattern_slots", ctypes.c_uint64)]
+                ("device", ctypes.c_int32), ("learned_pattern_slots", ctypes.c_uint64),
+                ("pattern_device_pages", ctypes.c_uint64)]
 
 
 class _EiDesc(ctypes.Structure):
@@ -134,6 +135,8 @@ def _load() -> ctypes.CDLL:
     L.snn_batch_net_counts.argtypes = [P, P, P]
     L.snn_net_set_profiling.argtypes = [P, I]
     L.snn_net_kernel_times.argtypes = [P, P, P, I, ctypes.c_char_p, U64]
+    L.snn_net_pattern_paging.argtypes = [P, P, P, P]
+    L.snn_net_pattern_paging.restype = I
     L.snn_graph_create.argtypes = [ctypes.POINTER(_Desc), I, ctypes.POINTER(P)]
     L.snn_graph_destroy.argtypes = [P]
     L.snn_graph_max_delay.argtypes = [P, P]
@@ -197,10 +200,10 @@ class Network:
 
     def __init__(self, net=None, *, max_events_per_tick: int = 0, fire_log_capacity: int = 0,
                  raster_capacity: int = 0, track_pre_count: bool = True, device: int = 0, spec=None,
-                 learned_pattern_slots: int = 0):
+                 learned_pattern_slots: int = 0, pattern_device_pages: int = 0):
         L = lib()
         o = Options(max_events_per_tick, fire_log_capacity, raster_capacity, int(track_pre_count), device,
-                    learned_pattern_slots)
+                    learned_pattern_slots, pattern_device_pages)
         self.track_pre_count = track_pre_count
         if spec is not None:
             self._create_from_spec(spec, o)
@@ -279,6 +282,13 @@ class Network:
         if batch:
             arr = np.array(batch, dtype=np.float64)
             self.inject(arr[:, 0].astype(np.uint32), arr[:, 1].astype(np.uint64), arr[:, 2].astype(np.float32))
+
+    def pattern_paging(self) -> Dict[str, int]:
+        """The learned-pattern pool's pages, the device cache's pages (0: the whole pool on the device)
+        and the page copies made so far."""
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().snn_net_pattern_paging(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return dict(pages=a.value, device_pages=b.value, moves=c.value)
 
     def counts(self):
         d, f, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
@@ -526,11 +536,11 @@ class Batch:
 
     def __init__(self, nets=None, *, max_events_per_tick: int = 0, fire_log_capacity: int = 0,
                  raster_capacity: int = 0, track_pre_count: bool = True, device: int = 0, specs=None,
-                 learned_pattern_slots: int = 0):
+                 learned_pattern_slots: int = 0, pattern_device_pages: int = 0):
         """``nets``: workloads.Net objects (host arrays, snn_batch_create); or ``specs``: ei_network
         specs generated on the device (snn_batch_create_ei)."""
         o = Options(max_events_per_tick, fire_log_capacity, raster_capacity, int(track_pre_count), device,
-                    learned_pattern_slots)
+                    learned_pattern_slots, pattern_device_pages)
         descs, keep = [], []
         h = ctypes.c_void_p()
         if specs is not None:

@@ -84,6 +84,15 @@ struct Group {             // a run of the tick's due list
     uint32_t pad;
 };
 
+// The learned-pattern pool in pages of 2^kRefPageShift neurons when it is larger than the device
+// holds: every page's rings live in host memory, and the pages most events reach are copied into a
+// device cache between ticks; a page's entry says where its rings are now.
+constexpr uint32_t kRefPageShift = 10;
+struct RefPage {
+    Runs* runs;
+    RefMeta* meta;
+};
+
 // Everything one tick needs. Plain pointers so the same struct is a kernel argument.
 struct Step {
     // network
@@ -113,6 +122,8 @@ struct Step {
     Runs* ref_runs;              // learned patterns: ref_cap slots per neuron
     RefMeta* ref_meta;
     uint32_t ref_cap;
+    RefPage* ref_page;           // paged pool: where each page's rings are (null: ref_runs / ref_meta)
+    uint32_t* page_hits;         // paged pool: events per page since the last residency update
     uint32_t* last_fired;
     const float* decay_tab;      // 4 x kDecayTable
     const float* ltp;            // kStdpTable
@@ -173,6 +184,11 @@ struct Step {
 };
 
 SNN_HD RefsView refs_of(const Step& S, uint32_t x) {
+    if (S.ref_page) {
+        const RefPage pg = S.ref_page[x >> kRefPageShift];
+        const uint64_t b = (uint64_t)(x & ((1u << kRefPageShift) - 1u)) * S.ref_cap;
+        return RefsView{pg.runs + b, pg.meta + b, S.ref_cap};
+    }
     const uint64_t b = (uint64_t)x * S.ref_cap;
     return RefsView{S.ref_runs + b, S.ref_meta + b, S.ref_cap};
 }

@@ -855,6 +855,11 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
     const uint32_t T = tick_of(S);
     const uint32_t start = bucket_start(S, p);
     const uint32_t c = S.ptotal[p];
+    if (S.page_hits && threadIdx.x == 0 && c) {
+        // the events reaching each learned-pattern page of the partition (paged pool residency)
+        const uint32_t x0 = p << S.pshift, x1 = min(S.n, (p + 1) << S.pshift) - 1;
+        for (uint32_t q = x0 >> kRefPageShift; q <= (x1 >> kRefPageShift); ++q) atomicAdd(&S.page_hits[q], c);
+    }
     const E* src = (const E*)S.prec + start;
     const uint16_t* tls = S.ptl + start;
     constexpr uint32_t kNoTarget = 0xFFFFu;
@@ -1631,10 +1636,24 @@ struct snn_net {
     // fire-event recording (fixed-spike adjoint input): every inject delivered, in delivery order
     bool record = false;
     std::vector<PendingInject> inj_log;
+    // paged learned-pattern pool (Step::ref_page): every page's rings in pinned host memory, a device
+    // cache of cache_pages pages; slot_of_page[q] is page q's cache slot (-1: on the host), table
+    // mirrors the device page table
+    bool paged = false;
+    uint32_t npages = 0, cache_pages = 0;
+    Runs* host_runs = nullptr;
+    RefMeta* host_meta = nullptr;
+    Runs* cache_runs = nullptr;
+    RefMeta* cache_meta = nullptr;
+    std::vector<int32_t> slot_of_page, page_of_slot;
+    std::vector<RefPage> table;
+    uint64_t page_moves = 0;
 
     ~snn_net() {
         if (device >= 0) (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
+        if (host_runs) (void)hipHostFree(host_runs);
+        if (host_meta) (void)hipHostFree(host_meta);
         for (auto e : event_pool) (void)hipEventDestroy(e);
         for (auto& b : run_owned) (void)hipFree(b.p);
         for (auto& b : owned) (void)hipFree(b.p);
@@ -2157,13 +2176,50 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
         uint64_t want = 1;
         for (const auto& p : params) want = std::max<uint64_t>(want, p.max_refs);
         if (o.learned_pattern_slots) want = std::min<uint64_t>(want, o.learned_pattern_slots);
+        want = std::min<uint64_t>(want, kHeadMask);
         size_t free_b = 0, total_b = 0;
         check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
         const uint64_t per = sizeof(Runs) + sizeof(RefMeta);
-        const uint64_t fit = std::max<uint64_t>(1, total_b / 4 / ((uint64_t)std::max<uint32_t>(n, 1) * per));
-        S.ref_cap = (uint32_t)std::min<uint64_t>({want, fit, (uint64_t)kHeadMask});
-        S.ref_runs = dev_alloc<Runs>(own, (uint64_t)n * S.ref_cap);
-        S.ref_meta = dev_alloc<RefMeta>(own, (uint64_t)n * S.ref_cap);
+        const uint64_t budget = total_b / 4;
+        const uint32_t npages = (uint32_t)((n + (1u << kRefPageShift) - 1) >> kRefPageShift);
+        const uint64_t page_bytes = ((uint64_t)1 << kRefPageShift) * want * per;
+        uint64_t cache = o.pattern_device_pages;
+        if (cache == 0 && (uint64_t)n * want * per > budget) cache = std::max<uint64_t>(1, budget / page_bytes);
+        if (cache == 0 || cache >= npages) {
+            S.ref_cap = (uint32_t)want;
+            S.ref_runs = dev_alloc<Runs>(own, (uint64_t)n * S.ref_cap);
+            S.ref_meta = dev_alloc<RefMeta>(own, (uint64_t)n * S.ref_cap);
+        } else {
+            // every page in pinned host memory (the kernels reach it directly), `cache` pages also in a
+            // device cache; the run moves the pages most events reach into the cache between ticks
+            S.ref_cap = (uint32_t)want;
+            const uint64_t slots = (uint64_t)npages << kRefPageShift;
+            const uint64_t page_slots = (uint64_t)1 << kRefPageShift;
+            net->paged = true;
+            net->npages = npages;
+            net->cache_pages = (uint32_t)cache;
+            void* hr = nullptr;
+            void* hm = nullptr;
+            check(hipHostMalloc(&hr, slots * want * sizeof(Runs), hipHostMallocMapped), "hipHostMalloc (pattern pool)");
+            net->host_runs = (Runs*)hr;
+            check(hipHostMalloc(&hm, slots * want * sizeof(RefMeta), hipHostMallocMapped), "hipHostMalloc (pattern pool)");
+            net->host_meta = (RefMeta*)hm;
+            std::memset(hr, 0, slots * want * sizeof(Runs));
+            std::memset(hm, 0, slots * want * sizeof(RefMeta));
+            Runs* dr = nullptr;
+            RefMeta* dm = nullptr;
+            check(hipHostGetDevicePointer((void**)&dr, hr, 0), "hipHostGetDevicePointer");
+            check(hipHostGetDevicePointer((void**)&dm, hm, 0), "hipHostGetDevicePointer");
+            net->cache_runs = dev_alloc<Runs>(own, cache * page_slots * want);
+            net->cache_meta = dev_alloc<RefMeta>(own, cache * page_slots * want);
+            net->table.resize(npages);
+            for (uint32_t q = 0; q < npages; ++q)
+                net->table[q] = RefPage{dr + (uint64_t)q * page_slots * want, dm + (uint64_t)q * page_slots * want};
+            net->slot_of_page.assign(npages, -1);
+            net->page_of_slot.assign(cache, -1);
+            S.ref_page = dev_upload(own, net->table);
+            S.page_hits = dev_alloc<uint32_t>(own, npages);
+        }
     }
     // code pattern records: per source when every synapse has one code offset and every target the
     // same temporal parameters (SNNCUDA_CODE_PER_SYNAPSE=1 keeps them per synapse), else per synapse
@@ -2847,6 +2903,69 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 // run_until with the segment's buffers allocated per call (static_staging false), or staged in
 // `scratch` with nothing allocated or freed on the device (true: stream launches only, no HIP
 // graph, no autotuning, no profiling events)
+// Paged learned-pattern pool: the cache holds the pages most events reached since the last update
+// (ties: pages already held, then lower pages). Evicted pages are copied back to their host place and
+// loaded ones into the freed slots, on the step's stream before the next ticks; the page table
+// follows. Results do not depend on where a page is.
+void update_residency(snn_net* net) {
+    if (!net->paged) return;
+    Step& S = net->S;
+    const uint32_t np = net->npages, nc = net->cache_pages;
+    std::vector<uint32_t> hits(np);
+    check(hipMemcpyAsync(hits.data(), S.page_hits, (size_t)np * 4, hipMemcpyDeviceToHost, net->stream), "read page hits");
+    check(hipStreamSynchronize(net->stream), "hipStreamSynchronize");
+    std::vector<uint32_t> order(np);
+    for (uint32_t q = 0; q < np; ++q) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        if (hits[a] != hits[b]) return hits[a] > hits[b];
+        return (net->slot_of_page[a] >= 0) > (net->slot_of_page[b] >= 0);
+    });
+    std::vector<uint8_t> want(np, 0);
+    for (uint32_t k = 0; k < nc && k < np; ++k) want[order[k]] = 1;
+    const uint64_t page_slots = ((uint64_t)1 << kRefPageShift) * S.ref_cap;
+    auto host_runs = [&](uint32_t q) { return net->host_runs + (uint64_t)q * page_slots; };
+    auto host_meta = [&](uint32_t q) { return net->host_meta + (uint64_t)q * page_slots; };
+    Runs* dr = nullptr;
+    RefMeta* dm = nullptr;
+    check(hipHostGetDevicePointer((void**)&dr, net->host_runs, 0), "hipHostGetDevicePointer");
+    check(hipHostGetDevicePointer((void**)&dm, net->host_meta, 0), "hipHostGetDevicePointer");
+    bool moved = false;
+    for (uint32_t slot = 0; slot < nc; ++slot) {  // evictions
+        const int32_t q = net->page_of_slot[slot];
+        if (q < 0 || want[q]) continue;
+        check(hipMemcpyAsync(host_runs(q), net->cache_runs + slot * page_slots, page_slots * sizeof(Runs),
+                             hipMemcpyDeviceToHost, net->stream), "page out");
+        check(hipMemcpyAsync(host_meta(q), net->cache_meta + slot * page_slots, page_slots * sizeof(RefMeta),
+                             hipMemcpyDeviceToHost, net->stream), "page out");
+        net->table[q] = RefPage{dr + (uint64_t)q * page_slots, dm + (uint64_t)q * page_slots};
+        net->slot_of_page[q] = -1;
+        net->page_of_slot[slot] = -1;
+        moved = true;
+        ++net->page_moves;
+    }
+    uint32_t free_slot = 0;
+    for (uint32_t k = 0; k < nc && k < np; ++k) {  // loads into free slots
+        const uint32_t q = order[k];
+        if (net->slot_of_page[q] >= 0) continue;
+        while (net->page_of_slot[free_slot] >= 0) ++free_slot;
+        check(hipMemcpyAsync(net->cache_runs + free_slot * page_slots, host_runs(q), page_slots * sizeof(Runs),
+                             hipMemcpyHostToDevice, net->stream), "page in");
+        check(hipMemcpyAsync(net->cache_meta + free_slot * page_slots, host_meta(q), page_slots * sizeof(RefMeta),
+                             hipMemcpyHostToDevice, net->stream), "page in");
+        net->table[q] = RefPage{net->cache_runs + free_slot * page_slots, net->cache_meta + free_slot * page_slots};
+        net->slot_of_page[q] = (int32_t)free_slot;
+        net->page_of_slot[free_slot] = (int32_t)q;
+        moved = true;
+        ++net->page_moves;
+    }
+    if (moved)
+        check(hipMemcpyAsync(S.ref_page, net->table.data(), (size_t)np * sizeof(RefPage), hipMemcpyHostToDevice,
+                             net->stream), "page table");
+    check(hipMemsetAsync(S.page_hits, 0, (size_t)np * 4, net->stream), "clear page hits");
+    // the table and the host copies stay valid until the stream has used them
+    check(hipStreamSynchronize(net->stream), "hipStreamSynchronize");
+}
+
 int run_until_impl(snn_net* net, uint64_t inclusive_tick, bool static_staging) {
     if (!net) return fail(SNN_EINVAL, "null handle");
     if (inclusive_tick < net->next_tick) return SNN_OK;
@@ -2973,9 +3092,38 @@ int run_until_impl(snn_net* net, uint64_t inclusive_tick, bool static_staging) {
 
 extern "C" {
 
-int snn_net_run_until(snn_net* net, uint64_t inclusive_tick) { return run_until_impl(net, inclusive_tick, false); }
+// A paged learned-pattern pool updates its device cache every kPageEpoch ticks: the run is cut into
+// segments of that many ticks (cutting a run changes no result).
+constexpr uint64_t kPageEpoch = 64;
+int run_paged(snn_net* net, uint64_t inclusive_tick, bool static_staging) {
+    if (!net) return fail(SNN_EINVAL, "null handle");
+    if (!net->paged) return run_until_impl(net, inclusive_tick, static_staging);
+    while (net->next_tick <= inclusive_tick) {
+        const int rc = guarded([&]() -> int {
+            check(hipSetDevice(net->device), "hipSetDevice");
+            update_residency(net);
+            return SNN_OK;
+        });
+        if (rc != SNN_OK) return rc;
+        const uint64_t end = std::min(inclusive_tick, net->next_tick + kPageEpoch - 1);
+        const int r = run_until_impl(net, end, static_staging);
+        if (r != SNN_OK) return r;
+    }
+    return SNN_OK;
+}
 
-int snn_net_run_until_static(snn_net* net, uint64_t inclusive_tick) { return run_until_impl(net, inclusive_tick, true); }
+int snn_net_run_until(snn_net* net, uint64_t inclusive_tick) { return run_paged(net, inclusive_tick, false); }
+
+int snn_net_run_until_static(snn_net* net, uint64_t inclusive_tick) { return run_paged(net, inclusive_tick, true); }
+
+int snn_net_pattern_paging(snn_net* net, uint64_t* pages, uint64_t* device_pages, uint64_t* moves) {
+    if (!net) return fail(SNN_EINVAL, "null handle");
+    const uint64_t all = ((uint64_t)net->n + (1u << kRefPageShift) - 1) >> kRefPageShift;
+    if (pages) *pages = net->paged ? net->npages : all;
+    if (device_pages) *device_pages = net->paged ? net->cache_pages : 0;
+    if (moves) *moves = net->page_moves;
+    return SNN_OK;
+}
 
 int snn_net_counts(snn_net* net, uint64_t* delivered, uint64_t* fired, uint64_t* next_tick) {
     if (!net) return fail(SNN_EINVAL, "null handle");

@@ -37,11 +37,11 @@ def code_patterns(max_refs=5, fires=7):
     return _net([src, tgt], syn, cmds, "capacity_code_patterns")
 
 
-def many_learned(seed, n=300, ticks=2000, max_refs=(5, 12, 40, 500)):
+def many_learned(seed, n=300, ticks=2000, max_refs=(5, 12, 40, 500), rate=60):
     """Inject-driven neurons that learn far more than four temporal patterns: windows of at most 7
     ticks (so a candidate never holds more than 8 distinct offsets), several injects to one neuron in
     a tick, and max_reference_patterns up to the reference default 500. The only synapses come from
-    a silent neuron, so no code pattern is observed."""
+    a silent neuron, so no code pattern is observed. Up to `rate` injects per tick."""
     import numpy as np
     rng = np.random.default_rng(seed)
     params = [(float(rng.uniform(0.8, 1.5)), int(rng.choice([3, 5, 7])), float(rng.choice([0.9, 0.97, 0.9999])),
@@ -50,7 +50,7 @@ def many_learned(seed, n=300, ticks=2000, max_refs=(5, 12, 40, 500)):
     syn = [dict(src=n, tgt=int(t), w=0.1, plastic=False) for t in rng.integers(0, n, 50)]
     cmds = []
     for t in range(ticks):
-        for x in rng.integers(0, n, int(rng.integers(0, 60))):
+        for x in rng.integers(0, n, int(rng.integers(0, rate))):
             cmds.append((0, int(x), t, float(rng.uniform(0.1, 1.2))))
         if t % 50 == 49:
             cmds.append((1, 0, t, 0.0))

@@ -185,3 +185,27 @@ def test_neurons_holding_many_learned_patterns_match_the_oracle(seed):
         for k in ("membrane", "current", "spike_count", "t_match", "t_learned", "last_fired"):
             assert np.array_equal(dn[k], rn[k]), k
         assert np.array_equal(dev.raster(), ref.fires()[1])
+
+
+@pytest.mark.parametrize("device_pages,one_call", [(1, False), (2, False), (1, True)])
+def test_paged_pattern_pool_equals_the_device_pool_and_the_oracle(device_pages, one_call):
+    # 3,000 neurons (three 1,024-neuron pages) learning up to 500 patterns each; the paged runs hold
+    # one or two pages in the device cache and move pages at every run_until call and every 64 ticks
+    from tests import capacity_nets as C
+    net = C.many_learned(3, n=3000, ticks=600, rate=600)
+    if one_call:
+        net.commands = [c for c in net.commands if c[0] == 0] + [(1, 0, 599, 0.0)]
+    print(f"paged pool: device_pages {device_pages} one_call {one_call}", flush=True)
+    ref = oracle.Oracle(net)
+    ref.run_commands(net.commands)
+    with run_device(net) as a, run_device(net, pattern_device_pages=device_pages) as b:
+        pg = b.pattern_paging()
+        assert pg["pages"] == 3 and pg["device_pages"] == device_pages and pg["moves"] > 0, pg
+        assert a.pattern_paging()["device_pages"] == 0
+        na, nb, nr = a.neurons(), b.neurons(), ref.neurons()
+        for k in ("membrane", "current", "spike_count", "t_match", "t_learned", "last_fired"):
+            assert np.array_equal(na[k], nb[k]), k
+            assert np.array_equal(nb[k], nr[k]), k
+        assert np.array_equal(a.raster(), b.raster())
+        assert int(nr["t_learned"].max()) > 8
+
