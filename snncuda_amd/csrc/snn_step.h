@@ -726,12 +726,17 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
     return err;
 }
 
-// post-synaptic scan of neuron x after m fires at tick T, for one incoming record
+// post-synaptic scan of neuron x after m fires at tick T, for one incoming record r (a register
+// copy); returns whether r changed and must be stored
+SNN_HD bool post_scan_apply(const Step& S, StdpRec& r, uint32_t T, uint32_t m) {
+    if (r.last_post == T && (r.aux >> 8) == m) return false;  // done inside the fold
+    post_catch_up(S, r, T, m).store(r);
+    return true;
+}
+
 SNN_HD void post_scan_record(const Step& S, uint64_t t, uint32_t T, uint32_t m) {
     StdpRec r = S.stdp[t];
-    if (r.last_post == T && (r.aux >> 8) == m) return;  // done inside the fold
-    post_catch_up(S, r, T, m).store(r);
-    S.stdp[t] = r;
+    if (post_scan_apply(S, r, T, m)) S.stdp[t] = r;
 }
 
 }  // namespace snn

@@ -1025,7 +1025,9 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
     }
 }
 
-// post-synaptic scan: one wave per (first) fire of tick T
+// post-synaptic scan: one wave per (first) fire of tick T, kPostBatch records per lane loaded before
+// any is updated, so each lane has that many 32-byte records in flight
+constexpr uint32_t kPostBatch = 4;
 __global__ void __launch_bounds__(kBlock) k_post(Step S) {
     const uint32_t T = tick_of(S);
     const uint32_t R = S.lmax + 1;
@@ -1040,7 +1042,19 @@ __global__ void __launch_bounds__(kBlock) k_post(Step S) {
         if (!m) continue;
         const uint32_t x = S.firelog[idx];
         const uint64_t lo = S.in_ptr[x], hi = S.in_ptr[x + 1];
-        for (uint64_t t = lo + lane; t < hi; t += kWave) post_scan_record(S, t, T, m);
+        for (uint64_t t0 = lo + lane; t0 < hi; t0 += kPostBatch * kWave) {
+            StdpRec r[kPostBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kPostBatch; ++u) {
+                const uint64_t t = t0 + u * kWave;
+                if (t < hi) r[u] = S.stdp[t];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kPostBatch; ++u) {
+                const uint64_t t = t0 + u * kWave;
+                if (t < hi && post_scan_apply(S, r[u], T, m)) S.stdp[t] = r[u];
+            }
+        }
     }
 }
 
