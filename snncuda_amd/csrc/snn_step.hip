@@ -164,6 +164,27 @@ __device__ __forceinline__ void block_sum2(uint32_t& a, uint32_t& b) {
     __syncthreads();
 }
 
+// This block's base (the sum of the block partials before it) and their total, for up to 4 * kBlock
+// partials: each thread's loads are issued together.
+__device__ __forceinline__ void partials_prefix(const uint32_t* partials, uint32_t nparts, uint32_t& before,
+                                                uint32_t& all) {
+    static_assert(kScanBlocks <= 4 * kBlock, "four partials per thread");
+    uint32_t v[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t j = threadIdx.x + u * kBlock;
+        v[u] = j < nparts ? partials[j] : 0u;
+    }
+    before = 0;
+    all = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+        all += v[u];
+        before += threadIdx.x + u * kBlock < blockIdx.x ? v[u] : 0u;
+    }
+    block_sum2(before, all);
+}
+
 // Down-sweep of the scan. Each block forms its base from the block partials itself (the sum of
 // those before it) and block 0 the grand total: *total, and when cap is exceeded err_flag raised
 // and *zero_on_overflow and *total zeroed (so the tick does no further work).
@@ -172,13 +193,8 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(Src src, const uint32_t* p
                                                       uint64_t cap, uint32_t err_flag, uint32_t* err,
                                                       uint32_t* zero_on_overflow) {
     const uint32_t n = src.count();
-    uint32_t before = 0, all = 0;
-    for (uint32_t j = threadIdx.x; j < gridDim.x; j += blockDim.x) {
-        const uint32_t v = partials[j];
-        all += v;
-        before += j < blockIdx.x ? v : 0u;
-    }
-    block_sum2(before, all);
+    uint32_t before, all;
+    partials_prefix(partials, gridDim.x, before, all);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (total) *total = all;
         if (cap && all > cap) {
@@ -949,37 +965,55 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
     fold_rounds<BLOCK, WIDE, STDP, CODE_HERE, CODE2>(S, lds_stage, aux, off, c, start, T);
 }
 
+constexpr uint32_t kFireCodeBlocks = 32;  // k_fire_write's blocks for the per-source code patterns
 // Rank-ordered fire list of tick T into the fire ring (and the raster). kScanBlocks blocks, each over
 // a contiguous range of the fire bitmap's words, as k_scan_reduce summed them: a block's first
 // rank is the sum of the partials before it (the scan's down-sweep, done here), F their total.
 __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
     const uint32_t T = tick_of(S);
-    const uint32_t nwords = (S.seg_count[1] + 31u) >> 5;
-    uint32_t before = 0, all = 0;
-    for (uint32_t j = threadIdx.x; j < gridDim.x; j += blockDim.x) {
-        const uint32_t v = S.partials[j];
-        all += v;
-        before += j < blockIdx.x ? v : 0u;
+    if (blockIdx.x >= kScanBlocks) {
+        // the last kFireCodeBlocks blocks: the per-source code patterns of the fires of T - lmax,
+        // which have now reached every synapse (their ring entries are kept until this tick is
+        // written, so this runs beside the fire list)
+        if (!S.code_lazy || T < S.lmax) return;
+        const uint32_t te = T - S.lmax;
+        const uint32_t R = S.lmax + 1;
+        const uint32_t so = te % R;
+        const uint32_t Fo = S.tick_fcount[so];
+        const uint64_t st = S.tick_fstart[so];
+        uint32_t err = 0;
+        for (uint32_t f = (blockIdx.x - kScanBlocks) * blockDim.x + threadIdx.x; f < Fo; f += kFireCodeBlocks * blockDim.x) {
+            const uint64_t idx = (st + f) % S.fire_cap;
+            const uint32_t m = S.firemult[idx];
+            if (m) err |= code_observe2(S.src_code[S.firelog[idx]], te, m, S.tp_window, S.tp_sim, S.tp_max_refs);
+        }
+        if (err) atomicOr(&S.ctl[1], err);
+        return;
     }
-    block_sum2(before, all);
+    const uint32_t nwords = (S.seg_count[1] + 31u) >> 5;
+    const uint32_t chunk = (nwords + kScanBlocks - 1) / kScanBlocks;
+    const uint32_t lo = min(nwords, blockIdx.x * chunk);
+    const uint32_t hi = min(nwords, lo + chunk);
+    // the block's first bitmap word is loaded while the partials are summed
+    uint32_t next = lo + threadIdx.x < hi ? S.fire_bits[lo + threadIdx.x] : 0u;
+    uint32_t before, all;
+    partials_prefix(S.partials, kScanBlocks, before, all);
     const uint32_t F = all;
     const uint64_t head = S.totals[2];
     const uint32_t R = S.lmax + 1;
     // entries of ticks older than T+1-lmax may be overwritten (older than T-lmax with lazy code
-    // patterns, whose fires are applied below)
+    // patterns, whose fires the last blocks apply)
     const uint32_t keep = S.code_lazy ? S.lmax : S.lmax - 1;  // ticks kept besides T
     const uint64_t oldest = keep == 0 ? head : (T >= keep ? S.tick_fstart[(T - keep) % R] : 0);
     const bool fits = head + F - oldest <= S.fire_cap;
     const uint64_t rfill = S.totals[3];
     // the raster is a recording aid: when it is full, recording stops and the run goes on
     const bool rfits = S.raster && !(S.ctl[3] & 1u) && rfill + F <= S.raster_cap;
-    const uint32_t chunk = (nwords + gridDim.x - 1) / gridDim.x;
-    const uint32_t lo = min(nwords, blockIdx.x * chunk);
-    const uint32_t hi = min(nwords, lo + chunk);
     uint32_t base = before;
     for (uint32_t off = lo; off < hi; off += blockDim.x) {
         const uint32_t wi = off + threadIdx.x;
-        uint32_t bits = wi < hi ? S.fire_bits[wi] : 0u;
+        uint32_t bits = next;
+        next = wi + blockDim.x < hi ? S.fire_bits[wi + blockDim.x] : 0u;
         uint32_t tot;
         uint32_t pos = base + block_excl_scan(__popc(bits), &tot);
         base += tot;
@@ -1000,21 +1034,6 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
             }
             ++pos;
         }
-    }
-    if (S.code_lazy && T >= S.lmax) {
-        // the fires of T - lmax have reached every synapse: apply them to their sources' records
-        // (one entry per source and tick carries the multiplicity)
-        const uint32_t te = T - S.lmax;
-        const uint32_t so = te % R;
-        const uint32_t Fo = S.tick_fcount[so];
-        const uint64_t st = S.tick_fstart[so];
-        uint32_t err = 0;
-        for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < Fo; f += gridDim.x * blockDim.x) {
-            const uint64_t idx = (st + f) % S.fire_cap;
-            const uint32_t m = S.firemult[idx];
-            if (m) err |= code_observe2(S.src_code[S.firelog[idx]], te, m, S.tp_window, S.tp_sim, S.tp_max_refs);
-        }
-        if (err) atomicOr(&S.ctl[1], err);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         S.ctl[2] = F;
@@ -1885,7 +1904,7 @@ void launch_tick(snn_net* net, Timer& tm, bool graph = false) {
                        S.partials);
     tm.end();
     tm.begin(KFireWrite);
-    hipLaunchKernelGGL(k_fire_write, dim3(kScanBlocks), dim3(kBlock), 0, st, S);
+    hipLaunchKernelGGL(k_fire_write, dim3(kScanBlocks + kFireCodeBlocks), dim3(kBlock), 0, st, S);
     tm.end();
     if (net->stdp) {
         tm.begin(KPost);
