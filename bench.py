@@ -12,8 +12,9 @@ A step is one tick. Every run first steps SETTLE_TICKS ticks untimed (the activi
 ignition inject ramps up for ~100-200 ticks, then holds ~0.5% of the neurons firing per tick,
 ~1e7 delivered events per tick), then --warmup more ticks, then times exactly --steps ticks in one
 synchronous run_until call. `value` is delivered synaptic events per second over those ticks
-(input injects excluded). The same timed ticks carry HIP events around the scatter and the fold on
-the step's stream, so `roofline` and `value` describe the same ticks.
+(input injects excluded). Every PROFILE_EVERY-th of the same timed ticks carries HIP events around
+the scatter and the fold on the step's stream (events around every launch would slow the step by
+~5%), so `roofline` and `value` describe the same ticks.
 
 The JSON line carries:
   roofline       the dominant kernel (the partition fold): algorithmic bytes per launch = the
@@ -48,6 +49,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 SETTLE_TICKS = 300
+PROFILE_EVERY = 8  # timed ticks with kernel events: one in this many
 # bytes per delivered event of the plain step (STDP off), by the per-event byte model: static edge
 # fields 10 B (target, weight, delay, compartment/receptor) + event queue 8 B + synapse code-pattern
 # record read and write 32 B
@@ -137,8 +139,8 @@ def plain(args, stdp: bool) -> dict:
         first = SETTLE_TICKS + args.warmup
         dev.run_until(first - 1)
         # timed region: one synchronous call covering exactly --steps ticks, with HIP events around
-        # the scatter and the fold only
-        dev.set_profiling(True, kernels=("scatter", "fold"))
+        # the scatter and the fold on every PROFILE_EVERY-th tick
+        dev.set_profiling(True, kernels=("scatter", "fold"), every=PROFILE_EVERY)
         d0 = dev.delivered
         t0 = time.perf_counter()
         dev.run_until(first + args.steps - 1)
@@ -359,7 +361,7 @@ def spikecode(args) -> dict:
         dev.run_commands(injects)
         if first:
             dev.run_until(first - 1)
-        dev.set_profiling(True, kernels=("scatter", "fold"))
+        dev.set_profiling(True, kernels=("scatter", "fold"), every=PROFILE_EVERY)
         d0 = dev.delivered
         t0 = time.perf_counter()
         dev.run_until(T - 1)

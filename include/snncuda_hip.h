@@ -297,9 +297,9 @@ int snn_adjoint_backward(snn_adjoint* adj, const float* d_grad_trace, float* d_g
 
 /* Per-kernel time of the step, measured with HIP events on the handle's stream while profiling is
  * enabled (no counterpart in the reference). snn_net_set_profiling(enabled): 0 off, 1 every kernel,
- * 0x40000000 | mask only the kernels whose bit is set (bit k = the k-th name that
- * snn_net_kernel_times reports), so a timed run carries events around those kernels only; it also
- * clears the totals. snn_net_kernel_times: total ms and launches per kernel since then; names
+ * 0x40000000 | every << 16 | mask only the kernels whose bit is set (bit k = the k-th name that
+ * snn_net_kernel_times reports, mask < 2^16) and only on ticks that are a multiple of every (1..255;
+ * 0: every tick), so a timed run carries events around few launches; it also clears the totals. snn_net_kernel_times: total ms and launches per kernel since then; names
  * receives a comma-separated list. Returns the number of kernels written. */
 int snn_net_set_profiling(snn_net* net, int enabled);
 int snn_net_kernel_times(snn_net* net, double* ms_total, uint64_t* launches, int cap, char* names, uint64_t names_cap);

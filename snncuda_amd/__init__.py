@@ -393,13 +393,13 @@ class Network:
     KERNELS = ("plan", "segments", "scan_segments", "count", "scan_partitions", "scatter", "fold", "scan_fires",
                "fire_write", "post", "advance")
 
-    def set_profiling(self, on: bool, kernels=None) -> None:
-        """Time the step kernels with HIP events (all of them, or only those named in `kernels`);
-        clears the totals that kernel_times() reports."""
+    def set_profiling(self, on: bool, kernels=None, every: int = 1) -> None:
+        """Time the step kernels with HIP events (all of them, or only those named in `kernels`, on
+        every `every`-th tick); clears the totals that kernel_times() reports."""
         mode = int(bool(on))
-        if on and kernels is not None:
-            mode = 0x40000000
-            for k in kernels:
+        if on and (kernels is not None or every > 1):
+            mode = 0x40000000 | (min(max(int(every), 1), 255) << 16)
+            for k in (kernels if kernels is not None else self.KERNELS):
                 mode |= 1 << self.KERNELS.index(k)
         _check(lib().snn_net_set_profiling(self._h, mode))
 

@@ -1656,6 +1656,7 @@ struct snn_net {
     // profiling
     bool profiling = false;
     uint32_t prof_mask = ~0u;           // kernels timed (bit = KernelId)
+    uint32_t prof_every = 1;            // time them on every prof_every-th tick
     std::vector<hipEvent_t> event_pool;
     double kms[KCount_] = {0};
     uint64_t klaunch[KCount_] = {0};
@@ -1831,6 +1832,7 @@ struct Timer {
     void begin(int id) {
         cur = -1;
         if (!enabled || !net->profiling || !((net->prof_mask >> id) & 1u)) return;
+        if (net->prof_every > 1 && net->tick_next % net->prof_every != 0) return;
         cur = id;
         a = take();
         check(hipEventRecord(a, net->stream), "hipEventRecord");
@@ -3320,7 +3322,8 @@ int snn_net_read_plasticity(snn_net* net, uint64_t* ltp_updates, uint64_t* ltd_u
 int snn_net_set_profiling(snn_net* net, int enabled) {
     if (!net) return fail(SNN_EINVAL, "null handle");
     net->profiling = enabled != 0;
-    net->prof_mask = (enabled & kProfileSelect) ? (uint32_t)(enabled & ~kProfileSelect) : ~0u;
+    net->prof_mask = (enabled & kProfileSelect) ? (uint32_t)(enabled & 0xFFFF) : ~0u;
+    net->prof_every = (enabled & kProfileSelect) ? std::max(1u, ((uint32_t)enabled >> 16) & 0xFFu) : 1u;
     for (int k = 0; k < KCount_; ++k) {
         net->kms[k] = 0;
         net->klaunch[k] = 0;
