@@ -116,7 +116,9 @@ inline uint64_t partitions_of(uint64_t n, uint32_t pshift) { return (n + (1ull <
 // The default partition size for n neurons (2^10, 2^11 from 1,025 neurons, 2^12 from 512k), and the
 // error a network too large for one device raises when it is created.
 inline uint32_t default_partition_shift(uint64_t n) {
-    const uint32_t pshift = n <= 1024 ? 10 : (n < (1u << 19) ? 11 : 12);
+    // partitions of 1,024 neurons up to 2^18 neurons (at most 256 partitions: a network of 10^5
+    // neurons with 2,048-neuron partitions leaves most CUs without a fold workgroup), 2,048 up to 2^19
+    const uint32_t pshift = n <= (1u << 18) ? 10 : (n < (1u << 19) ? 11 : 12);
     if (partitions_of(n, 12) > kMaxPartitions)
         throw std::invalid_argument("network of " + std::to_string(n) + " neurons needs " +
                                     std::to_string(partitions_of(n, 12)) +

@@ -169,6 +169,8 @@ struct Step {
     uint32_t chunk;              // scatter write-combining chunk (records per partition buffer; 0: none)
     void* stage;                 // fold scratch for partitions larger than the LDS stage
     uint32_t* fire_bits;
+    uint32_t* fire_part;         // fires per block of the fire-list kernel (kFireBlocks), counted by the
+                                 // fold as it sets the fire bits (null: the caller ranks the bitmap)
     uint32_t* fire_neuron;
     uint32_t* fire_mult;
     uint32_t* fire_word_off;     // the CPU model's fire ranks (the device ranks fires in k_fire_write)
@@ -182,6 +184,23 @@ struct Step {
     uint32_t* fire_evt;          // optional: per fire position g, the index of the firing event in the
     uint32_t* raster_evt;        // target's ordered events of the tick; and per raster entry
 };
+
+// The fire-list kernel's blocks: the fire bitmap's ceil(D / 32) words of a tick with D events in
+// kFireBlocks contiguous ranges of ceil(words / kFireBlocks) words; fire_block is the block holding
+// fire position g.
+constexpr uint32_t kFireBlocks = 1024;
+SNN_HD uint32_t fire_block(uint32_t D, uint32_t g) {
+    const uint32_t nwords = (D + 31u) >> 5;
+    const uint32_t chunk = (nwords + kFireBlocks - 1) / kFireBlocks;
+    return (g >> 5) / chunk;
+}
+
+// A fire at position g: its bit in the fire bitmap, its block's fire count, its neuron.
+SNN_HD void mark_fire(const Step& S, uint32_t g, uint32_t x) {
+    SNN_ATOMIC_OR(&S.fire_bits[g >> 5], 1u << (g & 31u));
+    if (S.fire_part) SNN_ATOMIC_ADD(&S.fire_part[fire_block(S.seg_count[1], g)], 1u);
+    S.fire_neuron[g] = x;
+}
 
 SNN_HD RefsView refs_of(const Step& S, uint32_t x) {
     if (S.ref_page) {
@@ -573,8 +592,7 @@ SNN_HD uint32_t fold_piece(const Step& S, FoldTarget& f, E* ev, uint32_t kc, uin
             if ((f.taus_fires >> 8) == kFoldMaxFires) err |= kErrCounter;
             if ((f.taus_fires >> 8) == 0) f.first_g = g;
             f.taus_fires = ((f.taus_fires & ~0xFFu) | kResetTauCodes) + (1u << 8);
-            SNN_ATOMIC_OR(&S.fire_bits[g >> 5], 1u << (g & 31u));
-            S.fire_neuron[g] = x;
+            mark_fire(S, g, x);
             S.fire_mult[g] = 0;
             if (S.fire_evt) S.fire_evt[g] = f.k + i;
         }
@@ -695,8 +713,7 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
             d.cur[3] = 0.0f;
             d.taus = kResetTauCodes;
             d.spike_count += 1;
-            SNN_ATOMIC_OR(&S.fire_bits[g >> 5], 1u << (g & 31u));
-            S.fire_neuron[g] = x;
+            mark_fire(S, g, x);
             S.fire_mult[g] = 0;
             if (S.fire_evt) S.fire_evt[g] = i;
             if (fires == 0) first_g = g;

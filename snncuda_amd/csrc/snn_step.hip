@@ -59,6 +59,7 @@ void check(hipError_t e, const char* what) {
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kScanBlocks = 1024;
+static_assert(kScanBlocks == (int)kFireBlocks, "k_fire_write's blocks are the blocks the fold counts fires in");
 constexpr int kPBlock = 1024;  // partition count / scatter workgroups: 16 waves per CU
 // scatter workgroups at most: k_hscan scans four counts per thread of a 256-thread block, and the
 // fold's LDS table holds every workgroup's run start
@@ -74,12 +75,6 @@ struct PlainSrc {
     __device__ uint32_t load(uint32_t i) const { return p[i]; }
 };
 
-struct PopcSrc {  // popcount of the fire bitmap words covering the tick's events
-    const uint32_t* bits;
-    const uint32_t* events;
-    __device__ uint32_t count() const { return (*events + 31u) >> 5; }
-    __device__ uint32_t load(uint32_t i) const { return __popc(bits[i]); }
-};
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -288,6 +283,9 @@ __global__ void __launch_bounds__(kBlock) k_segments(Step S) {
     __shared__ Group lgrp[GLOBAL ? 1 : kMaxGroups];
     __shared__ uint32_t nseg_s;
     const Group* grp = GLOBAL ? S.groups : lgrp;
+    // the fire counts of this tick's fire-list blocks start at zero (the previous tick's fire-list
+    // kernel has read them)
+    if (threadIdx.x == 0) S.fire_part[blockIdx.x] = 0;
     if constexpr (!GLOBAL) {
         if (threadIdx.x < kWave) {
             if (S.fused_advance && blockIdx.x == 0 && threadIdx.x == 0 && T > S.inj_t0) advance_tick(S, T - 1);
@@ -967,8 +965,9 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
 
 constexpr uint32_t kFireCodeBlocks = 32;  // k_fire_write's blocks for the per-source code patterns
 // Rank-ordered fire list of tick T into the fire ring (and the raster). kScanBlocks blocks, each over
-// a contiguous range of the fire bitmap's words, as k_scan_reduce summed them: a block's first
-// rank is the sum of the partials before it (the scan's down-sweep, done here), F their total.
+// a contiguous range of the fire bitmap's words (fire_block): the fold counted each block's fires
+// as it set their bits, so a block's first rank is the sum of the counts before it and F their
+// total.
 __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
     const uint32_t T = tick_of(S);
     if (blockIdx.x >= kScanBlocks) {
@@ -997,7 +996,7 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
     // the block's first bitmap word is loaded while the partials are summed
     uint32_t next = lo + threadIdx.x < hi ? S.fire_bits[lo + threadIdx.x] : 0u;
     uint32_t before, all;
-    partials_prefix(S.partials, kScanBlocks, before, all);
+    partials_prefix(S.fire_part, kScanBlocks, before, all);
     const uint32_t F = all;
     const uint64_t head = S.totals[2];
     const uint32_t R = S.lmax + 1;
@@ -1619,6 +1618,7 @@ struct PendingInject {
 
 constexpr int kProfileSelect = 0x40000000;  // snn_net_set_profiling: time only the kernels in the mask
 enum KernelId { KPlan, KSegments, KScanSeg, KCount, KScanTgt, KScatter, KFold, KScanFire, KFireWrite, KPost, KAdvance, KCount_ };
+// "scan_fires" has no launch of its own: the fold counts the fire-list blocks' fires
 const char* kKernelNames[KCount_] = {"plan", "segments", "scan_segments", "count", "scan_partitions", "scatter",
                                      "fold", "scan_fires", "fire_write", "post", "advance"};
 
@@ -1900,10 +1900,6 @@ void launch_tick(snn_net* net, Timer& tm, bool graph = false) {
     tm.end();
     tm.begin(KFold);
     dispatch_fold(net, S);
-    tm.end();
-    tm.begin(KScanFire);
-    hipLaunchKernelGGL(k_scan_reduce<PopcSrc>, dim3(kScanBlocks), dim3(kBlock), 0, st, PopcSrc{S.fire_bits, &S.seg_count[1]},
-                       S.partials);
     tm.end();
     tm.begin(KFireWrite);
     hipLaunchKernelGGL(k_fire_write, dim3(kScanBlocks + kFireCodeBlocks), dim3(kBlock), 0, st, S);
@@ -2307,6 +2303,7 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     S.ptl = dev_alloc<uint16_t>(own, ev_cap, false);
     S.stage = net->wide ? (void*)dev_alloc<Entry16>(own, ev_cap, false) : (void*)dev_alloc<Entry8>(own, ev_cap, false);
     S.fire_bits = dev_alloc<uint32_t>(own, ev_cap / 32);
+    S.fire_part = dev_alloc<uint32_t>(own, kFireBlocks);
     S.fire_neuron = dev_alloc<uint32_t>(own, ev_cap, false);
     S.fire_mult = dev_alloc<uint32_t>(own, ev_cap, false);
     S.ctl = dev_alloc<uint32_t>(own, 4);

@@ -45,7 +45,7 @@ struct Emu {
     std::vector<uint32_t> seg_count;
     std::vector<uint64_t> seg_v0;
     std::vector<uint32_t> seg_n, seg_off, seg_mult;
-    std::vector<uint32_t> fire_bits, fire_neuron, fire_mult, fire_word_off, ctl;
+    std::vector<uint32_t> fire_bits, fire_part, fire_neuron, fire_mult, fire_word_off, ctl;
     std::vector<uint64_t> totals;
     std::vector<uint32_t> raster;
     std::vector<uint32_t> itgt, istage;
@@ -163,6 +163,7 @@ void tick(Emu& m, uint32_t t0) {
         D = 0;
     }
     S.seg_count[1] = D;
+    std::fill(m.fire_part.begin(), m.fire_part.end(), 0u);  // k_segments
     // count and scatter in a shuffled order
     std::vector<std::pair<uint32_t, uint32_t>> order;
     order.reserve(D);
@@ -182,6 +183,14 @@ void tick(Emu& m, uint32_t t0) {
     for (uint32_t w = 0; w < nwords; ++w) pc[w] = __builtin_popcount(S.fire_bits[w]);
     uint32_t F = 0;
     scan_excl(pc, nwords, m.fire_word_off, F);
+    {
+        // the fold's per-block fire counts equal the bitmap's popcounts over k_fire_write's blocks
+        // (an internal inconsistency is reported as an error flag no real error uses)
+        const uint32_t chunk = (nwords + kFireBlocks - 1) / kFireBlocks;
+        std::vector<uint32_t> want(kFireBlocks, 0);
+        for (uint32_t w = 0; w < nwords; ++w) want[w / chunk] += pc[w];
+        if (want != m.fire_part) S.ctl[1] |= 1u << 31;
+    }
     S.ctl[2] = F;
     // fire write (k_fire_write)
     const uint64_t head = S.totals[2];
@@ -359,6 +368,8 @@ void* emu_create(uint64_t n, const float* thr, const uint64_t* win, const float*
         m->fire_mult.assign(S.ev_cap, 0);
         m->fire_word_off.assign(S.ev_cap / 32 + 1, 0);
         S.fire_bits = m->fire_bits.data();
+        m->fire_part.assign(kFireBlocks, 0);
+        S.fire_part = m->fire_part.data();
         S.fire_neuron = m->fire_neuron.data();
         S.fire_mult = m->fire_mult.data();
         S.fire_word_off = m->fire_word_off.data();

@@ -77,7 +77,8 @@ def test_network_too_large_for_one_device_is_refused_at_create():
     # neurons. Past that, create raises a clear error instead of a launch failure at the first tick.
     from tests.emu import partition_shift
     assert partition_shift(1000)[0] == 10
-    assert partition_shift(100_000)[0] == 11
+    assert partition_shift(100_000)[0] == 10
+    assert partition_shift(300_000)[0] == 11
     assert partition_shift(1_010_000)[0] == 12
     assert partition_shift(8192 * 4096)[0] == 12
     r, msg = partition_shift(8192 * 4096 + 1)
