@@ -147,6 +147,7 @@ struct Step {
     uint64_t* seg_v0;
     uint32_t* seg_n;
     uint32_t* seg_off;
+    uint32_t* seg_loc;           // device: first event relative to the segment's k_segments block
     uint32_t* seg_mult;
     // target partitions: partition p holds neurons [p << pshift, (p+1) << pshift)
     uint32_t pshift, nparts, nwg;

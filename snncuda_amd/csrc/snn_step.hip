@@ -230,6 +230,9 @@ __device__ void advance_tick(const Step& S, uint32_t T) {
     S.ctl[0] = T + 1;
 }
 
+// k_segments' split of the tick's nseg segments: block b holds [b * chunk, (b + 1) * chunk)
+__device__ __forceinline__ uint32_t seg_chunk(uint32_t nseg) { return (nseg + kScanBlocks - 1) / kScanBlocks; }
+
 constexpr uint32_t kMaxGroups = 2 * 127 + 1;  // group table held in LDS: 2 * lmax + 1 with lmax <= 127
 
 // The tick's delivery-order groups and their first segments, by one wave, into grp[0..ng) (LDS or
@@ -303,30 +306,35 @@ __global__ void __launch_bounds__(kBlock) k_segments(Step S) {
     }
     __syncthreads();
     const uint32_t nseg = nseg_s;
-    // the range of the scan's block blockIdx.x (k_scan_down splits the segments the same way)
-    const uint32_t chunk = (nseg + gridDim.x - 1) / gridDim.x;
+    // the block's contiguous range of segments (seg_first gives k_pcount the same split), in rounds
+    // of blockDim.x: their descriptors, and each one's first event relative to the block's first
+    const uint32_t chunk = seg_chunk(nseg);
     const uint32_t lo = min(nseg, blockIdx.x * chunk);
     const uint32_t hi = min(nseg, lo + chunk);
-    uint32_t sum = 0;
-    for (uint32_t q = lo + threadIdx.x; q < hi; q += blockDim.x) {
-        // last group whose first segment is <= q (groups are in order, empty ones share `first`)
-        uint32_t a = 0, b = ng;
-        while (a < b) {
-            const uint32_t mid = (a + b) >> 1;
-            if (grp[mid].first <= q) a = mid + 1; else b = mid;
+    uint32_t run = 0;
+    for (uint32_t q0 = lo; q0 < hi; q0 += blockDim.x) {
+        const uint32_t q = q0 + threadIdx.x;
+        uint32_t cnt = 0;
+        if (q < hi) {
+            // last group whose first segment is <= q (groups are in order, empty ones share `first`)
+            uint32_t a = 0, b = ng;
+            while (a < b) {
+                const uint32_t mid = (a + b) >> 1;
+                if (grp[mid].first <= q) a = mid + 1; else b = mid;
+            }
+            const Group g = grp[a - 1];
+            uint64_t v0;
+            uint32_t mult;
+            segment_info(S, g, q - g.first, v0, cnt, mult);
+            S.seg_v0[q] = v0;
+            S.seg_mult[q] = mult;
         }
-        const Group g = grp[a - 1];
-        uint64_t v0;
-        uint32_t cnt, mult;
-        segment_info(S, g, q - g.first, v0, cnt, mult);
-        S.seg_v0[q] = v0;
-        S.seg_n[q] = cnt;
-        S.seg_mult[q] = mult;
-        sum += cnt;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(cnt, &tot);
+        if (q < hi) S.seg_loc[q] = run + ex;
+        run += tot;
     }
-    uint32_t tot;
-    block_excl_scan(sum, &tot);
-    if (threadIdx.x == 0) S.partials[blockIdx.x] = tot;
+    if (threadIdx.x == 0) S.partials[blockIdx.x] = run;
 }
 
 // Event ranges: workgroup w owns events [D*w/nwg, D*(w+1)/nwg) (a contiguous range of g), caches the
@@ -373,31 +381,82 @@ __device__ __forceinline__ uint32_t seg_lookup(const SegCache& c, uint32_t ga, u
 }
 
 // Per-workgroup histogram of the tick's events over target partitions (LDS atomics only).
+//
+// The tick's segment offsets have no scan kernel of their own: k_segments left each segment's first
+// event relative to its block (seg_loc) and each block's event count (partials). Every workgroup
+// forms the exclusive prefix of the block counts in LDS (pre), so segment q's first event is
+// pre[q / chunk] + seg_loc[q]; together the workgroups write those offsets to seg_off for the
+// scatter, and block 0 writes the tick's event count (or, past the event capacity, raises the
+// error and zeroes the tick's segments and events, so nothing further is done).
 __global__ void __launch_bounds__(kPBlock) k_pcount(Step S) {
+    static_assert(kPBlock == kScanBlocks, "one block count per thread");
     extern __shared__ uint32_t h[];
     __shared__ SegCache cache;
+    __shared__ uint32_t pre[kScanBlocks + 1];
+    const uint32_t nseg_all = S.seg_count[0];
+    uint32_t total;
+    {
+        const uint32_t ex = block_excl_scan_n<kPBlock>(S.partials[threadIdx.x], &total);
+        pre[threadIdx.x] = ex;
+        if (threadIdx.x == 0) pre[kScanBlocks] = total;
+    }
+    const uint64_t cap = S.ev_cap < kMaxEventsPerTick ? S.ev_cap : (uint64_t)kMaxEventsPerTick;
+    const bool over = total > cap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (over) {
+            atomicOr(&S.ctl[1], (uint32_t)kErrEvents);
+            S.seg_count[0] = 0;
+            S.seg_count[1] = 0;
+        } else {
+            S.seg_count[1] = total;
+        }
+    }
+    const uint32_t nseg = over ? 0u : nseg_all;
+    const uint64_t D = over ? 0u : total;
+    const uint32_t chunk = seg_chunk(nseg_all);
     for (uint32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) h[p] = 0;
     __syncthreads();
+    // this workgroup's share of the scatter's segment offsets
+    {
+        const uint32_t qe = (uint32_t)((uint64_t)nseg * (blockIdx.x + 1) / gridDim.x);
+        for (uint32_t q = (uint32_t)((uint64_t)nseg * blockIdx.x / gridDim.x) + threadIdx.x; q < qe; q += blockDim.x)
+            S.seg_off[q] = pre[q / chunk] + S.seg_loc[q];
+    }
     const uint32_t shift = S.pshift;
     // kCountUnroll events per thread at a time: their target loads are all in flight before the
     // histogram updates
     constexpr uint32_t kCountUnroll = 4;
-    const uint32_t nseg = S.seg_count[0];
-    const uint64_t D = S.seg_count[1];
     const uint32_t g0 = (uint32_t)(D * blockIdx.x / gridDim.x);
     const uint32_t g1 = (uint32_t)(D * (blockIdx.x + 1) / gridDim.x);
     __shared__ uint32_t qstart;
-    if (threadIdx.x == 0 && g0 < g1) qstart = seg_find(S.seg_off, nseg, g0);
+    if (threadIdx.x == 0 && g0 < g1) {
+        // the last segment whose first event is <= g0: in the last block (of those holding
+        // segments) whose first event is <= g0, the last segment whose offset is <= g0
+        const uint32_t nblk = (nseg + chunk - 1) / chunk;
+        uint32_t a = 1, b = nblk;  // pre[0] = 0 <= g0
+        while (a < b) {
+            const uint32_t mid = (a + b) >> 1;
+            if (pre[mid] <= g0) a = mid + 1; else b = mid;
+        }
+        const uint32_t blk = a - 1;
+        const uint32_t r = g0 - pre[blk];
+        uint32_t x = blk * chunk + 1, y = min(nseg, blk * chunk + chunk);  // seg_loc of the first is 0
+        while (x < y) {
+            const uint32_t mid = (x + y) >> 1;
+            if (S.seg_loc[mid] <= r) x = mid + 1; else y = mid;
+        }
+        qstart = x - 1;
+    }
     __syncthreads();
     uint32_t qa = g0 < g1 ? qstart : nseg;
     while (g0 < g1) {
         const uint32_t qb = min(nseg, qa + kSegChunk);
         const uint32_t nq = qb - qa;
         for (uint32_t k = threadIdx.x; k < nq; k += blockDim.x) {
-            cache.off[k] = S.seg_off[qa + k];
+            cache.off[k] = pre[(qa + k) / chunk] + S.seg_loc[qa + k];
             cache.v0[k] = S.seg_v0[qa + k];
         }
-        if (threadIdx.x == 0) cache.off[nq] = qb < nseg ? S.seg_off[qb] : (uint32_t)D;
+        if (threadIdx.x == 0) cache.off[nq] = qb < nseg ? pre[qb / chunk] + S.seg_loc[qb] : (uint32_t)D;
         __syncthreads();
         const uint32_t ga = max(g0, cache.off[0]);
         const uint32_t gb = min(g1, cache.off[nq]);
@@ -1618,7 +1677,8 @@ struct PendingInject {
 
 constexpr int kProfileSelect = 0x40000000;  // snn_net_set_profiling: time only the kernels in the mask
 enum KernelId { KPlan, KSegments, KScanSeg, KCount, KScanTgt, KScatter, KFold, KScanFire, KFireWrite, KPost, KAdvance, KCount_ };
-// "scan_fires" has no launch of its own: the fold counts the fire-list blocks' fires
+// "scan_segments" and "scan_fires" have no launches of their own: k_pcount forms the segment
+// offsets, and the fold counts the fire-list blocks' fires
 const char* kKernelNames[KCount_] = {"plan", "segments", "scan_segments", "count", "scan_partitions", "scatter",
                                      "fold", "scan_fires", "fire_write", "post", "advance"};
 
@@ -1884,11 +1944,6 @@ void launch_tick(snn_net* net, Timer& tm, bool graph = false) {
         hipLaunchKernelGGL(k_segments<false>, dim3(kScanBlocks), dim3(kBlock), 0, st, S);
         tm.end();
     }
-    tm.begin(KScanSeg);
-    hipLaunchKernelGGL(k_scan_down<PlainSrc>, dim3(kScanBlocks), dim3(kBlock), 0, st,
-                       PlainSrc{S.seg_n, &S.seg_count[0], 0}, (const uint32_t*)S.partials, S.seg_off, &S.seg_count[1],
-                       std::min<uint64_t>(S.ev_cap, kMaxEventsPerTick), (uint32_t)kErrEvents, &S.ctl[1], &S.seg_count[0]);
-    tm.end();
     tm.begin(KCount);
     hipLaunchKernelGGL(k_pcount, dim3(S.nwg), dim3(kPBlock), S.nparts * 4, st, S);
     tm.end();
@@ -2275,7 +2330,7 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     S.seg_cap = seg_cap;
     S.ev_cap = ev_cap;
     S.seg_v0 = dev_alloc<uint64_t>(own, seg_cap, false);
-    S.seg_n = dev_alloc<uint32_t>(own, seg_cap, false);
+    S.seg_loc = dev_alloc<uint32_t>(own, seg_cap, false);  // (seg_n is the CPU model's)
     S.seg_off = dev_alloc<uint32_t>(own, seg_cap, false);
     S.seg_mult = dev_alloc<uint32_t>(own, seg_cap, false);
     // Target partitions of 2048 neurons (4096 from 512k neurons): few enough buckets that the
