@@ -116,6 +116,8 @@ struct Step {
     uint32_t tp_uniform;         // every neuron has the same window / similarity / max_refs
     uint32_t tp_window, tp_max_refs;
     float tp_sim;
+    uint32_t thr_uniform;        // every neuron has the threshold thr_all
+    float thr_all;
     NeuronDyn* dyn;
     TemporalHdr* th;
     Runs* cand;
@@ -504,14 +506,15 @@ SNN_HD StdpTally post_catch_up(const Step& S, StdpRec& r, uint32_t T, uint32_t f
 // One target's fold within a tick, kept in registers while its events arrive in pieces: every
 // piece holds events of larger g than the pieces before it (the fold reads a partition's bucket
 // in runs of whole scatter workgroups, and each scatter workgroup owns a contiguous range of g).
-// Only what the tick's events change is carried: the compartments decayed to T, the tau codes and
-// the fire count; the rest of the neuron state is re-read at fold_end.
+// Everything fold_end writes to the neuron's dynamics is carried (the compartments decayed to T, the
+// tau codes, this tick's fires and the stored spike count), so the dynamics are read once per tick.
 struct FoldTarget {
     float cur[4];          // compartment currents, decayed to T at fold_begin
     uint32_t taus_fires;   // bits 0-7 receptor code per compartment, bits 8-31 fires this tick
     uint32_t k;            // events applied so far this tick
     uint32_t first_g;      // g of the first fire
     float threshold;
+    uint32_t spikes;       // spike count before this tick
 };
 constexpr uint32_t kFoldMaxFires = (1u << 24) - 1;
 
@@ -527,7 +530,8 @@ SNN_HD void fold_begin(const Step& S, FoldTarget& f, uint32_t x, uint32_t T) {
     f.taus_fires = d.taus & 0xFFu;
     f.k = 0;
     f.first_g = 0;
-    f.threshold = S.np[x].threshold;
+    f.threshold = S.thr_uniform ? S.thr_all : S.np[x].threshold;
+    f.spikes = d.spike_count;
 }
 
 // The next kc events of target x at tick T (ev: this piece, any order; sorted by g here), applied
@@ -606,15 +610,14 @@ SNN_HD uint32_t fold_piece(const Step& S, FoldTarget& f, E* ev, uint32_t kc, uin
 // membrane, so they are applied together), then the state written back. Nothing is written for a
 // target without events. The synapses whose events came before one of this tick's fires are
 // brought up to every fire by the post-synaptic scan kernel (post_scan_record). fold_end_load
-// reads the target's stored dynamics and temporal header (so a thread can issue the loads of all
-// its targets before it applies any), fold_end_apply finishes.
-SNN_HD void fold_end_load(const Step& S, const FoldTarget& f, uint32_t x, NeuronDyn& d, TemporalHdr& h) {
+// reads the target's temporal header (so a thread can issue the loads of all its targets before it
+// applies any), fold_end_apply finishes.
+SNN_HD void fold_end_load(const Step& S, const FoldTarget& f, uint32_t x, TemporalHdr& h) {
     if (f.k == 0) return;
-    d = S.dyn[x];
     h = S.th[x];
 }
 
-SNN_HD uint32_t fold_end_apply(const Step& S, const FoldTarget& f, uint32_t x, uint32_t T, NeuronDyn d, TemporalHdr h) {
+SNN_HD uint32_t fold_end_apply(const Step& S, const FoldTarget& f, uint32_t x, uint32_t T, TemporalHdr h) {
     if (f.k == 0) return 0;
     NeuronParams p;
     if (S.tp_uniform) {
@@ -631,13 +634,14 @@ SNN_HD uint32_t fold_end_apply(const Step& S, const FoldTarget& f, uint32_t x, u
         S.fire_mult[f.first_g] = fires;
         S.last_fired[x] = T;
     }
+    NeuronDyn d;
     d.cur[0] = f.cur[0];
     d.cur[1] = f.cur[1];
     d.cur[2] = f.cur[2];
     d.cur[3] = f.cur[3];
     d.membrane = f.cur[0] + f.cur[1] + f.cur[2] + f.cur[3];
     d.last_tick = T;
-    d.spike_count += fires;
+    d.spike_count = f.spikes + fires;
     d.taus = f.taus_fires & 0xFFu;
     S.dyn[x] = d;
     S.th[x] = h;
@@ -645,10 +649,9 @@ SNN_HD uint32_t fold_end_apply(const Step& S, const FoldTarget& f, uint32_t x, u
 }
 
 SNN_HD uint32_t fold_end(const Step& S, const FoldTarget& f, uint32_t x, uint32_t T) {
-    NeuronDyn d{};
     TemporalHdr h{};
-    fold_end_load(S, f, x, d, h);
-    return fold_end_apply(S, f, x, T, d, h);
+    fold_end_load(S, f, x, h);
+    return fold_end_apply(S, f, x, T, h);
 }
 
 // The sequential fold of target x's k events at tick T (ev is the target's own array, in any

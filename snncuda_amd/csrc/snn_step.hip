@@ -880,18 +880,17 @@ __device__ __forceinline__ void fold_chunks(const Step& S, E* lds_stage, uint32_
         lo = nlo;
         hi = nhi;
     }
-    // the stored state of all the thread's targets is loaded before any is finished
-    NeuronDyn d[TPT];
+    // the temporal headers of all the thread's targets are loaded before any is finished
     TemporalHdr h[TPT];
 #pragma unroll
     for (uint32_t j = 0; j < TPT; ++j) {
         const uint32_t x = (p << S.pshift) + threadIdx.x + j * BLOCK;
-        if (j < tpt && x < S.n) fold_end_load(S, f[j], x, d[j], h[j]);
+        if (j < tpt && x < S.n) fold_end_load(S, f[j], x, h[j]);
     }
 #pragma unroll
     for (uint32_t j = 0; j < TPT; ++j) {
         const uint32_t x = (p << S.pshift) + threadIdx.x + j * BLOCK;
-        if (j < tpt && x < S.n) err |= fold_end_apply(S, f[j], x, T, d[j], h[j]);
+        if (j < tpt && x < S.n) err |= fold_end_apply(S, f[j], x, T, h[j]);
     }
     if (err) atomicOr(&S.ctl[1], err);
 }
@@ -2255,6 +2254,13 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     S.tp_window = params[0].window;
     S.tp_max_refs = params[0].max_refs;
     S.tp_sim = params[0].sim_thr;
+    S.thr_uniform = 1;
+    for (const auto& p : params)
+        if (std::memcmp(&p.threshold, &params[0].threshold, 4) != 0) {
+            S.thr_uniform = 0;
+            break;
+        }
+    S.thr_all = params[0].threshold;
     // learned-pattern pool: as many slots per neuron as the largest max_reference_patterns (or the
     // option), within a quarter of the device's memory; a neuron that learns beyond its slots
     // reports SNN_ECAPACITY

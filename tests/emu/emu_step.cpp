@@ -312,6 +312,10 @@ void* emu_create(uint64_t n, const float* thr, const uint64_t* win, const float*
         S.tp_window = h.params[0].window;
         S.tp_max_refs = h.params[0].max_refs;
         S.tp_sim = h.params[0].sim_thr;
+        S.thr_uniform = 1;
+        for (const auto& q : h.params)
+            if (std::memcmp(&q.threshold, &h.params[0].threshold, 4) != 0) S.thr_uniform = 0;
+        S.thr_all = h.params[0].threshold;
         m->dyn.assign(h.n, NeuronDyn{});
         m->th.assign(h.n, TemporalHdr{});
         m->cand.assign(h.n, Runs{});
