@@ -99,6 +99,7 @@ struct Step {
     uint32_t n, lmax;
     const uint64_t* row;
     const uint32_t* grp;
+    const uint64_t* gstart;      // optional: row[s] + grp[s * (lmax + 1) + j], the groups' absolute starts
     const uint32_t* ve;          // null when VE == slot
     const uint32_t* tgt;
     const float* w;
@@ -280,6 +281,12 @@ SNN_HD void segment_info(const Step& S, const Group& g, uint32_t r, uint64_t& v0
     mult = S.firemult[fi];
     const uint32_t L = g.b;
     const uint64_t base = (uint64_t)s * (S.lmax + 1);
+    if (S.gstart) {
+        // one random 16-byte read instead of the row start and the group offsets
+        v0 = S.gstart[base + L - 1];
+        cnt = (uint32_t)(S.gstart[base + L] - v0);
+        return;
+    }
     const uint32_t lo = S.grp[base + L - 1];
     const uint32_t hi = S.grp[base + L];
     v0 = S.row[s] + lo;

@@ -1138,6 +1138,13 @@ __global__ void __launch_bounds__(kBlock) k_post(Step S) {
 // launches, whose earlier ticks are finished by the next tick's k_segments)
 __global__ void k_advance(Step S) { advance_tick(S, tick_of(S)); }
 
+// Absolute start of every (source, lag) group, row[s] + grp[s * W + j] (k_segments reads one of
+// these pairs per segment instead of the row start and the group offsets).
+__global__ void k_group_starts(const uint64_t* row, const uint32_t* grp, uint64_t n, uint32_t W, uint64_t* out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n * W; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = row[i / W] + grp[i];
+}
+
 // the STDP records' update counters, record order
 __global__ void k_stdp_counts(const StdpRec* stdp, uint64_t nt, uint32_t* upd, uint32_t* ltp) {
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < nt; t += (uint64_t)gridDim.x * blockDim.x) {
@@ -2244,6 +2251,19 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     auto& own = net->owned;
     Step& S = net->S;
     const uint32_t n = S.n;
+    {
+        // the groups' absolute starts, when they take at most 1/16 of the free device memory
+        const uint64_t words = (uint64_t)n * (S.lmax + 1);
+        size_t free_b = 0, total_b = 0;
+        check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        if (words * 8 <= free_b / 16) {
+            uint64_t* gs = dev_alloc<uint64_t>(own, words, false);
+            hipLaunchKernelGGL(k_group_starts, dim3(2048), dim3(kBlock), 0, net->stream, S.row, S.grp, (uint64_t)n,
+                               S.lmax + 1, gs);
+            check(hipGetLastError(), "k_group_starts");
+            S.gstart = gs;
+        }
+    }
     S.tp_uniform = 1;
     for (const auto& p : params)
         if (p.window != params[0].window || p.max_refs != params[0].max_refs ||
