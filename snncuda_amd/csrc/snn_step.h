@@ -157,6 +157,7 @@ struct Step {
     uint32_t* hist;              // [nparts][nwg] per-workgroup event counts, then their offsets within
                                  // the partition's bucket
     uint32_t* ptotal;            // [nparts] events per partition (bucket p starts at the sum before it)
+    uint32_t* pstart;            // [nparts] bucket starts (the scatter's first workgroup writes them)
     uint32_t* partials;          // block partial sums of the two-kernel scans (kScanBlocks)
     // the tick being launched (kNone: read ctl[0], as HIP graph replays do), and whether the first
     // kernel of the tick also finishes the previous tick's counters (stream launches)

@@ -509,22 +509,6 @@ __global__ void __launch_bounds__(kBlock) k_hscan(Step S) {
     if (threadIdx.x == 0) S.ptotal[p] = tot;
 }
 
-// Exclusive prefix of the partition totals up to partition p (the start of bucket p), summed by the
-// whole block.
-__device__ __forceinline__ uint32_t bucket_start(const Step& S, uint32_t p) {
-    __shared__ uint32_t red[kPBlock / kWave];
-    uint32_t s = 0;
-    for (uint32_t q = threadIdx.x; q < p; q += blockDim.x) s += S.ptotal[q];
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_down(s, o, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = s;
-    __syncthreads();
-    uint32_t t = 0;
-    for (uint32_t w = 0; w < blockDim.x / kWave; ++w) t += red[w];
-    __syncthreads();
-    return t;
-}
-
 // The block's events in rounds of EPT * blockDim.x consecutive positions, EPT per thread: every
 // thread runs the same number of rounds (f gets ok = false past the end), so f may contain block
 // barriers.
@@ -617,6 +601,7 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
             if (p < np) {
                 base[p] = ex + S.hist[(uint64_t)p * S.nwg + blockIdx.x];
                 cnt[p] = 0;
+                if (blockIdx.x == 0) S.pstart[p] = ex;  // for the fold
             }
             run += tot;
         }
@@ -925,7 +910,7 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
     constexpr uint32_t kScanPer = 1024 / BLOCK;
     const uint32_t p = blockIdx.x;
     const uint32_t T = tick_of(S);
-    const uint32_t start = bucket_start(S, p);
+    const uint32_t start = S.pstart[p];
     const uint32_t c = S.ptotal[p];
     if (S.page_hits && threadIdx.x == 0 && c) {
         // the events reaching each learned-pattern page of the partition (paged pool residency)
@@ -2394,6 +2379,7 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     if (!net->partials) net->partials = dev_alloc<uint32_t>(own, kScanBlocks);
     S.partials = net->partials;
     S.ptotal = dev_alloc<uint32_t>(own, net->hist_cap_parts);
+    S.pstart = dev_alloc<uint32_t>(own, net->hist_cap_parts);
     S.tick = kNone;
     S.fused_advance = 0;
     int cus = 256;
