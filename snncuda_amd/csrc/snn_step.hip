@@ -781,15 +781,6 @@ __device__ __forceinline__ void fold_chunks(const Step& S, E* lds_stage, uint32_
     const E* src = (const E*)S.prec + start;
     const uint16_t* tls = S.ptl + start;
     for (uint32_t t = threadIdx.x; t < 2 * (PS + 1); t += BLOCK) off[t] = 0;
-    // thread's targets: threadIdx.x + j * BLOCK, so the state loads and stores are coalesced
-    FoldTarget f[TPT];
-#pragma unroll
-    for (uint32_t j = 0; j < TPT; ++j) {
-        const uint32_t x = (p << S.pshift) + threadIdx.x + j * BLOCK;
-        f[j].k = 0;
-        if (j < tpt && x < S.n) fold_begin(S, f[j], x, T);
-    }
-    __syncthreads();
     // End of the chunk that starts at the run start lo: the last run start within lo + kCap (no run
     // exceeds kCap, so it is past lo). runs[0] = 0 <= lo and runs[nwg] = c keep the search in range.
     auto chunk_end = [&](uint32_t lo) -> uint32_t {
@@ -811,12 +802,22 @@ __device__ __forceinline__ void fold_chunks(const Step& S, E* lds_stage, uint32_
             if (i < b) load_words(src + i, ev[u]);
         }
     };
-    uint32_t err = 0;
-    uint32_t* oc = off;           // counts, then places, of the current chunk
-    uint32_t* on = off + PS + 1;  // the next chunk's counts
+    // the first chunk's records are requested before the targets' state, so both round trips overlap
     uint32_t lo = 0;
     uint32_t hi = chunk_end(0);
     load(lo, hi);
+    // thread's targets: threadIdx.x + j * BLOCK, so the state loads and stores are coalesced
+    FoldTarget f[TPT];
+#pragma unroll
+    for (uint32_t j = 0; j < TPT; ++j) {
+        const uint32_t x = (p << S.pshift) + threadIdx.x + j * BLOCK;
+        f[j].k = 0;
+        if (j < tpt && x < S.n) fold_begin(S, f[j], x, T);
+    }
+    __syncthreads();
+    uint32_t err = 0;
+    uint32_t* oc = off;           // counts, then places, of the current chunk
+    uint32_t* on = off + PS + 1;  // the next chunk's counts
     while (lo < c) {
 #pragma unroll
         for (uint32_t u = 0; u < R; ++u)
@@ -1032,6 +1033,15 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
         if (err) atomicOr(&S.ctl[1], err);
         return;
     }
+    // the ring and raster state is loaded first, so its round trip overlaps the others
+    const uint64_t head = S.totals[2];
+    const uint64_t rfill = S.totals[3];
+    const uint32_t rflags = S.ctl[3];
+    const uint32_t R = S.lmax + 1;
+    // entries of ticks older than T+1-lmax may be overwritten (older than T-lmax with lazy code
+    // patterns, whose fires the last blocks apply)
+    const uint32_t keep = S.code_lazy ? S.lmax : S.lmax - 1;  // ticks kept besides T
+    const uint64_t oldest = keep == 0 ? head : (T >= keep ? S.tick_fstart[(T - keep) % R] : 0);
     const uint32_t nwords = (S.seg_count[1] + 31u) >> 5;
     const uint32_t chunk = (nwords + kScanBlocks - 1) / kScanBlocks;
     const uint32_t lo = min(nwords, blockIdx.x * chunk);
@@ -1041,16 +1051,9 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
     uint32_t before, all;
     partials_prefix(S.fire_part, kScanBlocks, before, all);
     const uint32_t F = all;
-    const uint64_t head = S.totals[2];
-    const uint32_t R = S.lmax + 1;
-    // entries of ticks older than T+1-lmax may be overwritten (older than T-lmax with lazy code
-    // patterns, whose fires the last blocks apply)
-    const uint32_t keep = S.code_lazy ? S.lmax : S.lmax - 1;  // ticks kept besides T
-    const uint64_t oldest = keep == 0 ? head : (T >= keep ? S.tick_fstart[(T - keep) % R] : 0);
     const bool fits = head + F - oldest <= S.fire_cap;
-    const uint64_t rfill = S.totals[3];
     // the raster is a recording aid: when it is full, recording stops and the run goes on
-    const bool rfits = S.raster && !(S.ctl[3] & 1u) && rfill + F <= S.raster_cap;
+    const bool rfits = S.raster && !(rflags & 1u) && rfill + F <= S.raster_cap;
     uint32_t base = before;
     for (uint32_t off = lo; off < hi; off += blockDim.x) {
         const uint32_t wi = off + threadIdx.x;
