@@ -76,6 +76,11 @@ typedef struct snn_options {
                                        as that quarter holds; n > 0 gives a device cache of n pages
                                        (the whole pool on the device when n covers the network).
                                        Results do not depend on where the pages are. */
+    uint64_t pattern_runs;          /* runs (distinct offsets, run-length encoded) a neuron's temporal
+                                       pattern holds, its candidate and each learned copy (0: 8; at
+                                       most 64). A neuron that learns a candidate with more distinct
+                                       offsets in its window fails the run with SNN_ECAPACITY; each
+                                       run costs 8 B per neuron and per learned-pattern slot. */
 } snn_options;
 
 typedef struct snn_net snn_net;

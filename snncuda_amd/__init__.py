@@ -56,7 +56,7 @@ class Options(ctypes.Structure):
     _fields_ = [("max_events_per_tick", ctypes.c_uint64), ("fire_log_capacity", ctypes.c_uint64),
                 ("raster_capacity", ctypes.c_uint64), ("track_pre_count", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("learned_pattern_slots", ctypes.c_uint64),
-                ("pattern_device_pages", ctypes.c_uint64)]
+                ("pattern_device_pages", ctypes.c_uint64), ("pattern_runs", ctypes.c_uint64)]
 
 
 class _EiDesc(ctypes.Structure):
@@ -200,10 +200,10 @@ class Network:
 
     def __init__(self, net=None, *, max_events_per_tick: int = 0, fire_log_capacity: int = 0,
                  raster_capacity: int = 0, track_pre_count: bool = True, device: int = 0, spec=None,
-                 learned_pattern_slots: int = 0, pattern_device_pages: int = 0):
+                 learned_pattern_slots: int = 0, pattern_device_pages: int = 0, pattern_runs: int = 0):
         L = lib()
         o = Options(max_events_per_tick, fire_log_capacity, raster_capacity, int(track_pre_count), device,
-                    learned_pattern_slots, pattern_device_pages)
+                    learned_pattern_slots, pattern_device_pages, pattern_runs)
         self.track_pre_count = track_pre_count
         if spec is not None:
             self._create_from_spec(spec, o)
@@ -536,11 +536,11 @@ class Batch:
 
     def __init__(self, nets=None, *, max_events_per_tick: int = 0, fire_log_capacity: int = 0,
                  raster_capacity: int = 0, track_pre_count: bool = True, device: int = 0, specs=None,
-                 learned_pattern_slots: int = 0, pattern_device_pages: int = 0):
+                 learned_pattern_slots: int = 0, pattern_device_pages: int = 0, pattern_runs: int = 0):
         """``nets``: workloads.Net objects (host arrays, snn_batch_create); or ``specs``: ei_network
         specs generated on the device (snn_batch_create_ei)."""
         o = Options(max_events_per_tick, fire_log_capacity, raster_capacity, int(track_pre_count), device,
-                    learned_pattern_slots, pattern_device_pages)
+                    learned_pattern_slots, pattern_device_pages, pattern_runs)
         descs, keep = [], []
         h = ctypes.c_void_p()
         if specs is not None:

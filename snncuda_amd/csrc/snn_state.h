@@ -38,7 +38,10 @@ namespace snn {
 constexpr uint32_t kNone = 0xFFFFFFFFu;      // unset tick (std::nullopt)
 constexpr uint32_t kDecayTable = 16384;      // expf(-dt/tau) is exactly 0 beyond this for tau <= 150
 constexpr uint32_t kStdpTable = 4096;        // expf(-dt/20) is exactly 0 beyond this
-constexpr uint32_t kRuns = 8;                // run-length capacity of a neuron temporal pattern
+// run-length capacity of a neuron temporal pattern (the candidate and each learned copy): kRuns
+// unless the network asks for more (snn_options.pattern_runs), at most kRunsMax
+constexpr uint32_t kRuns = 8;
+constexpr uint32_t kRunsMax = 64;
 constexpr uint32_t kHeadMask = 0x00FFFFFFu;  // TemporalHdr::head_last: learned-pattern ring head
 constexpr uint32_t kLastMatch = 1u << 24;     // TemporalHdr::head_last: the last observation matched
 constexpr uint32_t kCodeMax = 8;             // max offsets in a synapse code pattern
@@ -98,9 +101,11 @@ struct TemporalHdr {       // 32 B
     uint32_t head_last;    // bits 0-23 ring head (kHeadMask), bit 24 last_match (kLastMatch)
 };
 
-struct Runs {              // 64 B, run-length encoded non-decreasing offsets
-    uint32_t val[kRuns];
-    uint32_t cnt[kRuns];
+// A temporal pattern is run-length encoded (its offsets are non-decreasing): an array of runs, as
+// many as the network's run capacity; a run's value and count share a cache line.
+struct Run {
+    uint32_t val;          // the offset
+    uint32_t cnt;          // how many consecutive offsets equal it
 };
 
 struct RefMeta {           // a learned pattern's length and stored runs
@@ -110,17 +115,19 @@ struct RefMeta {           // a learned pattern's length and stored runs
 
 // One neuron's learned patterns (learn_pattern's FIFO of at most max_reference_patterns copies,
 // SynapseDendriteProcessor.cpp:31-41) as a ring of `cap` slots in the network's pattern pool:
-// neuron x owns slots [x * cap, (x + 1) * cap) of the Runs and RefMeta arrays.
+// neuron x owns slots [x * cap, (x + 1) * cap) of the run and RefMeta arrays.
 struct RefsView {
-    Runs* ref;
+    Run* ref;              // slot s's runs: ref + s * runs
     RefMeta* meta;
     uint32_t cap;
+    uint32_t runs;         // run capacity of a pattern (and of the neuron's candidate)
+    SNN_HD Run* at(uint32_t slot) const { return ref + (uint64_t)slot * runs; }
 };
 
 // cosine over run-length encoded sequences, prefix of k = min(len_l, len_r) entries.
 // TemporalPatternMatcher::cosine_similarity (src/snn/TemporalPattern.cpp:35-60): the FP64 sums of
 // integer products are exact, so summing per run equals summing per index.
-SNN_HD float cosine_runs(const Runs& l, uint32_t ln_runs, uint32_t l_len, const Runs& r,
+SNN_HD float cosine_runs(const Run* l, uint32_t ln_runs, uint32_t l_len, const Run* r,
                          uint32_t rn_runs, uint32_t r_len) {
     if (l_len == 0 || r_len == 0) return 0.0f;
     uint32_t k = l_len < r_len ? l_len : r_len;
@@ -129,19 +136,19 @@ SNN_HD float cosine_runs(const Runs& l, uint32_t ln_runs, uint32_t l_len, const 
     (void)ln_runs;
     (void)rn_runs;
     while (k > 0) {
-        const uint32_t la = l.cnt[i] - li;
-        const uint32_t ra = r.cnt[j] - rj;
+        const uint32_t la = l[i].cnt - li;
+        const uint32_t ra = r[j].cnt - rj;
         uint32_t take = la < ra ? la : ra;
         if (take > k) take = k;
-        const uint64_t lv = l.val[i], rv = r.val[j];
+        const uint64_t lv = l[i].val, rv = r[j].val;
         dot += (uint64_t)take * lv * rv;
         lnorm += (uint64_t)take * lv * lv;
         rnorm += (uint64_t)take * rv * rv;
         k -= take;
         li += take;
         rj += take;
-        if (li == l.cnt[i]) { ++i; li = 0; }
-        if (rj == r.cnt[j]) { ++j; rj = 0; }
+        if (li == l[i].cnt) { ++i; li = 0; }
+        if (rj == r[j].cnt) { ++j; rj = 0; }
     }
     if (lnorm == 0 || rnorm == 0) return 0.0f;
     const double d = (double)dot, a = (double)lnorm, b = (double)rnorm;
@@ -149,20 +156,20 @@ SNN_HD float cosine_runs(const Runs& l, uint32_t ln_runs, uint32_t l_len, const 
 }
 
 // cosine_runs with the right sequence's run `ov` counted as `ov_cnt` (a count held in a register)
-SNN_HD float cosine_runs_ov(const Runs& l, uint32_t l_len, const Runs& r, uint32_t r_len, uint32_t ov,
+SNN_HD float cosine_runs_ov(const Run* l, uint32_t l_len, const Run* r, uint32_t r_len, uint32_t ov,
                             uint32_t ov_cnt) {
     if (l_len == 0 || r_len == 0) return 0.0f;
     uint32_t k = l_len < r_len ? l_len : r_len;
     uint64_t dot = 0, lnorm = 0, rnorm = 0;
     uint32_t i = 0, j = 0, li = 0, rj = 0;
     while (k > 0) {
-        const uint32_t lc = l.cnt[i];
-        const uint32_t rc = j == ov ? ov_cnt : r.cnt[j];
+        const uint32_t lc = l[i].cnt;
+        const uint32_t rc = j == ov ? ov_cnt : r[j].cnt;
         const uint32_t la = lc - li;
         const uint32_t ra = rc - rj;
         uint32_t take = la < ra ? la : ra;
         if (take > k) take = k;
-        const uint64_t lv = l.val[i], rv = r.val[j];
+        const uint64_t lv = l[i].val, rv = r[j].val;
         dot += (uint64_t)take * lv * rv;
         lnorm += (uint64_t)take * lv * lv;
         rnorm += (uint64_t)take * rv * rv;
@@ -178,9 +185,9 @@ SNN_HD float cosine_runs_ov(const Runs& l, uint32_t l_len, const Runs& r, uint32
 }
 
 // Prefix length covered by the stored runs of a candidate.
-SNN_HD uint32_t stored_len(const Runs& c, uint32_t nruns) {
+SNN_HD uint32_t stored_len(const Run* c, uint32_t nruns) {
     uint32_t s = 0;
-    for (uint32_t i = 0; i < nruns; ++i) s += c.cnt[i];
+    for (uint32_t i = 0; i < nruns; ++i) s += c[i].cnt;
     return s;
 }
 
@@ -205,12 +212,12 @@ SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, const RefsView& 
     const bool repeat = !reset && last_match && h.n >= h.maxlen;
     // append v
     if (!h.trunc) {
-        if (h.nruns > 0 && cand->val[h.nruns - 1] == v) {
-            cand->cnt[h.nruns - 1] += 1;
+        if (h.nruns > 0 && cand[h.nruns - 1].val == v) {
+            cand[h.nruns - 1].cnt += 1;
             cand_dirty = true;
-        } else if (h.nruns < kRuns) {
-            cand->val[h.nruns] = v;
-            cand->cnt[h.nruns] = 1;
+        } else if (h.nruns < refs.runs) {
+            cand[h.nruns].val = v;
+            cand[h.nruns].cnt = 1;
             h.nruns += 1;
             cand_dirty = true;
         } else {
@@ -225,7 +232,7 @@ SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, const RefsView& 
         match = false;
     } else {
         // every compared prefix must be inside the stored runs
-        const uint32_t have = h.trunc ? stored_len(*cand, h.nruns) : h.n;
+        const uint32_t have = h.trunc ? stored_len(cand, h.nruns) : h.n;
         float best = 0.0f;
         const uint32_t head = h.head_last & kHeadMask;
         for (uint32_t q = 0; q < h.nlearned; ++q) {
@@ -236,7 +243,7 @@ SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, const RefsView& 
                 err |= kErrTemporal;
                 continue;
             }
-            const float c = cosine_runs(refs.ref[slot], refs.meta[slot].nruns, rl, *cand, h.nruns, need);
+            const float c = cosine_runs(refs.at(slot), refs.meta[slot].nruns, rl, cand, h.nruns, need);
             best = (best < c) ? c : best;
         }
         match = best >= p.sim_thr;
@@ -258,8 +265,8 @@ SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, const RefsView& 
     if (h.nlearned >= refs.cap) return err | kErrTemporal;
     const uint32_t slot = (head + h.nlearned) % refs.cap;
     for (uint32_t i = 0; i < h.nruns; ++i) {
-        refs.ref[slot].val[i] = cand->val[i];
-        refs.ref[slot].cnt[i] = cand->cnt[i];
+        refs.at(slot)[i].val = cand[i].val;
+        refs.at(slot)[i].cnt = cand[i].cnt;
     }
     refs.meta[slot].len = h.n;
     refs.meta[slot].nruns = h.nruns;
@@ -293,8 +300,8 @@ SNN_HD uint32_t temporal_observe(TemporalHdr& h, CandPtr& cand, const RefsView& 
 template <typename CandPtr>
 SNN_HD uint32_t temporal_restart_tick(TemporalHdr& h, CandPtr cand, const RefsView& refs, uint32_t times,
                                       const NeuronParams& p) {
-    cand->val[0] = 0;
-    cand->cnt[0] = times;
+    cand[0].val = 0;
+    cand[0].cnt = times;
     h.nruns = 1;
     h.n = times;
     h.head_last &= ~kLastMatch;
@@ -303,8 +310,8 @@ SNN_HD uint32_t temporal_restart_tick(TemporalHdr& h, CandPtr cand, const RefsVi
     const uint32_t head0 = h.head_last & kHeadMask;
     uint32_t err = 0, head = head0, nl;
     auto learn = [&](uint32_t slot, uint32_t len) {
-        refs.ref[slot].val[0] = 0;
-        refs.ref[slot].cnt[0] = len;
+        refs.at(slot)[0].val = 0;
+        refs.at(slot)[0].cnt = len;
         refs.meta[slot].len = len;
         refs.meta[slot].nruns = 1;
     };
@@ -347,15 +354,16 @@ SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, const RefsVi
         if (!(0.0f >= p.sim_thr)) return temporal_restart_tick(h, cand, refs, times, p);
     }
     const uint32_t v = tick - h.start;
-    // the run this tick's offsets go to (kRuns: none, the candidate is truncated)
-    uint32_t run = kRuns, cnt = 0;
+    // the run this tick's offsets go to (R: none, the candidate is truncated)
+    const uint32_t R = refs.runs;
+    uint32_t run = R, cnt = 0;
     if (!h.trunc) {
-        if (h.nruns > 0 && cand->val[h.nruns - 1] == v) {
+        if (h.nruns > 0 && cand[h.nruns - 1].val == v) {
             run = h.nruns - 1;
-            cnt = cand->cnt[run];
-        } else if (h.nruns < kRuns) {
+            cnt = cand[run].cnt;
+        } else if (h.nruns < R) {
             run = h.nruns;
-            cand->val[run] = v;
+            cand[run].val = v;
             cnt = 0;
             h.nruns += 1;
         } else {
@@ -370,14 +378,14 @@ SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, const RefsVi
             const uint32_t rest = times - t;
             h.n += rest;
             h.match += rest;
-            if (run < kRuns) cnt += rest;
+            if (run < R) cnt += rest;
             break;
         }
-        if (run < kRuns) cnt += 1;
+        if (run < R) cnt += 1;
         h.n += 1;
         bool match = false;
         if (h.nlearned != 0 && h.n >= 2) {
-            const uint32_t have = h.trunc ? stored_len(*cand, h.nruns) : h.n;
+            const uint32_t have = h.trunc ? stored_len(cand, h.nruns) : h.n;
             float best = 0.0f;
             const uint32_t head = h.head_last & kHeadMask;
             for (uint32_t q = 0; q < h.nlearned; ++q) {
@@ -388,7 +396,7 @@ SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, const RefsVi
                     err |= kErrTemporal;
                     continue;
                 }
-                const float c = cosine_runs_ov(refs.ref[slot], rl, *cand, need, run, cnt);
+                const float c = cosine_runs_ov(refs.at(slot), rl, cand, need, run, cnt);
                 best = (best < c) ? c : best;
             }
             match = best >= p.sim_thr;
@@ -405,7 +413,7 @@ SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, const RefsVi
             err |= kErrTemporal;
             continue;
         }
-        if (run < kRuns) cand->cnt[run] = cnt;
+        if (run < R) cand[run].cnt = cnt;
         uint32_t head = h.head_last & kHeadMask;
         if (h.nlearned >= p.max_refs) {
             head = (head + 1) % refs.cap;
@@ -417,8 +425,8 @@ SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, const RefsVi
         }
         const uint32_t slot = (head + h.nlearned) % refs.cap;
         for (uint32_t i = 0; i < h.nruns; ++i) {
-            refs.ref[slot].val[i] = cand->val[i];
-            refs.ref[slot].cnt[i] = cand->cnt[i];
+            refs.at(slot)[i].val = cand[i].val;
+            refs.at(slot)[i].cnt = cand[i].cnt;
         }
         refs.meta[slot].len = h.n;
         refs.meta[slot].nruns = h.nruns;
@@ -431,7 +439,7 @@ SNN_HD uint32_t temporal_observe_tick(TemporalHdr& h, CandPtr cand, const RefsVi
         }
         h.maxlen = ml;
     }
-    if (run < kRuns) cand->cnt[run] = cnt;
+    if (run < R) cand[run].cnt = cnt;
     return err;
 }
 

@@ -89,7 +89,7 @@ struct Group {             // a run of the tick's due list
 // device cache between ticks; a page's entry says where its rings are now.
 constexpr uint32_t kRefPageShift = 10;
 struct RefPage {
-    Runs* runs;
+    Run* runs;
     RefMeta* meta;
 };
 
@@ -121,8 +121,9 @@ struct Step {
     float thr_all;
     NeuronDyn* dyn;
     TemporalHdr* th;
-    Runs* cand;
-    Runs* ref_runs;              // learned patterns: ref_cap slots per neuron
+    Run* cand;                   // run_cap runs per neuron
+    Run* ref_runs;               // learned patterns: ref_cap slots of run_cap runs per neuron
+    uint32_t run_cap;            // runs per temporal pattern (1 .. kRunsMax; kRuns by default)
     RefMeta* ref_meta;
     uint32_t ref_cap;
     RefPage* ref_page;           // paged pool: where each page's rings are (null: ref_runs / ref_meta)
@@ -211,10 +212,10 @@ SNN_HD RefsView refs_of(const Step& S, uint32_t x) {
     if (S.ref_page) {
         const RefPage pg = S.ref_page[x >> kRefPageShift];
         const uint64_t b = (uint64_t)(x & ((1u << kRefPageShift) - 1u)) * S.ref_cap;
-        return RefsView{pg.runs + b, pg.meta + b, S.ref_cap};
+        return RefsView{pg.runs + b * S.run_cap, pg.meta + b, S.ref_cap, S.run_cap};
     }
     const uint64_t b = (uint64_t)x * S.ref_cap;
-    return RefsView{S.ref_runs + b, S.ref_meta + b, S.ref_cap};
+    return RefsView{S.ref_runs + b * S.run_cap, S.ref_meta + b, S.ref_cap, S.run_cap};
 }
 
 SNN_HD uint32_t lower_bound_u32(const uint32_t* a, uint64_t lo, uint64_t hi, uint32_t v) {
@@ -636,7 +637,7 @@ SNN_HD uint32_t fold_end_apply(const Step& S, const FoldTarget& f, uint32_t x, u
         p = S.np[x];
     }
     p.threshold = f.threshold;
-    const uint32_t err = temporal_observe_tick(h, &S.cand[x], refs_of(S, x), T, f.k, p);
+    const uint32_t err = temporal_observe_tick(h, S.cand + (uint64_t)x * S.run_cap, refs_of(S, x), T, f.k, p);
     const uint32_t fires = f.taus_fires >> 8;
     if (fires) {
         S.fire_mult[f.first_g] = fires;
@@ -674,7 +675,7 @@ SNN_HD uint32_t fold_events(const Step& S, E* ev, uint32_t k, uint32_t x, uint32
     NeuronDyn d = S.dyn[x];
     TemporalHdr h = S.th[x];
     uint32_t err = 0;
-    Runs* cand = &S.cand[x];
+    Run* cand = S.cand + (uint64_t)x * S.run_cap;
     decay_to(d, T, S.decay_tab);
     uint32_t fires = 0, first_g = 0;
     for (uint32_t i = 0; i < k; ++i) {

@@ -1729,9 +1729,9 @@ struct snn_net {
     // mirrors the device page table
     bool paged = false;
     uint32_t npages = 0, cache_pages = 0;
-    Runs* host_runs = nullptr;
+    Run* host_runs = nullptr;
     RefMeta* host_meta = nullptr;
-    Runs* cache_runs = nullptr;
+    Run* cache_runs = nullptr;
     RefMeta* cache_meta = nullptr;
     std::vector<int32_t> slot_of_page, page_of_slot;
     std::vector<RefPage> table;
@@ -2224,7 +2224,9 @@ void open_net(snn_net* net, const snn_options& o, uint32_t n, uint64_t e, uint32
     S.pre_count = o.track_pre_count ? dev_alloc<uint32_t>(own, e) : nullptr;
     S.dyn = dev_alloc<NeuronDyn>(own, n);
     S.th = dev_alloc<TemporalHdr>(own, n);
-    S.cand = dev_alloc<Runs>(own, n);
+    if (o.pattern_runs > kRunsMax) throw std::invalid_argument("pattern_runs above " + std::to_string(kRunsMax));
+    S.run_cap = o.pattern_runs ? (uint32_t)o.pattern_runs : kRuns;
+    S.cand = dev_alloc<Run>(own, (uint64_t)n * S.run_cap);
     S.last_fired = dev_alloc<uint32_t>(own, n, false);
     check(hipMemset(S.last_fired, 0xFF, (size_t)n * 4), "hipMemset");
     S.decay_tab = dev_upload(own, host_decay_table());
@@ -2279,7 +2281,8 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
         want = std::min<uint64_t>(want, kHeadMask);
         size_t free_b = 0, total_b = 0;
         check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-        const uint64_t per = sizeof(Runs) + sizeof(RefMeta);
+        const uint64_t R = S.run_cap;
+        const uint64_t per = R * sizeof(Run) + sizeof(RefMeta);
         const uint64_t budget = total_b / 4;
         const uint32_t npages = (uint32_t)((n + (1u << kRefPageShift) - 1) >> kRefPageShift);
         const uint64_t page_bytes = ((uint64_t)1 << kRefPageShift) * want * per;
@@ -2287,7 +2290,7 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
         if (cache == 0 && (uint64_t)n * want * per > budget) cache = std::max<uint64_t>(1, budget / page_bytes);
         if (cache == 0 || cache >= npages) {
             S.ref_cap = (uint32_t)want;
-            S.ref_runs = dev_alloc<Runs>(own, (uint64_t)n * S.ref_cap);
+            S.ref_runs = dev_alloc<Run>(own, (uint64_t)n * S.ref_cap * R);
             S.ref_meta = dev_alloc<RefMeta>(own, (uint64_t)n * S.ref_cap);
         } else {
             // every page in pinned host memory (the kernels reach it directly), `cache` pages also in a
@@ -2300,21 +2303,21 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
             net->cache_pages = (uint32_t)cache;
             void* hr = nullptr;
             void* hm = nullptr;
-            check(hipHostMalloc(&hr, slots * want * sizeof(Runs), hipHostMallocMapped), "hipHostMalloc (pattern pool)");
-            net->host_runs = (Runs*)hr;
+            check(hipHostMalloc(&hr, slots * want * R * sizeof(Run), hipHostMallocMapped), "hipHostMalloc (pattern pool)");
+            net->host_runs = (Run*)hr;
             check(hipHostMalloc(&hm, slots * want * sizeof(RefMeta), hipHostMallocMapped), "hipHostMalloc (pattern pool)");
             net->host_meta = (RefMeta*)hm;
-            std::memset(hr, 0, slots * want * sizeof(Runs));
+            std::memset(hr, 0, slots * want * R * sizeof(Run));
             std::memset(hm, 0, slots * want * sizeof(RefMeta));
-            Runs* dr = nullptr;
+            Run* dr = nullptr;
             RefMeta* dm = nullptr;
             check(hipHostGetDevicePointer((void**)&dr, hr, 0), "hipHostGetDevicePointer");
             check(hipHostGetDevicePointer((void**)&dm, hm, 0), "hipHostGetDevicePointer");
-            net->cache_runs = dev_alloc<Runs>(own, cache * page_slots * want);
+            net->cache_runs = dev_alloc<Run>(own, cache * page_slots * want * R);
             net->cache_meta = dev_alloc<RefMeta>(own, cache * page_slots * want);
             net->table.resize(npages);
             for (uint32_t q = 0; q < npages; ++q)
-                net->table[q] = RefPage{dr + (uint64_t)q * page_slots * want, dm + (uint64_t)q * page_slots * want};
+                net->table[q] = RefPage{dr + (uint64_t)q * page_slots * want * R, dm + (uint64_t)q * page_slots * want};
             net->slot_of_page.assign(npages, -1);
             net->page_of_slot.assign(cache, -1);
             S.ref_page = dev_upload(own, net->table);
@@ -3025,9 +3028,10 @@ void update_residency(snn_net* net) {
     std::vector<uint8_t> want(np, 0);
     for (uint32_t k = 0; k < nc && k < np; ++k) want[order[k]] = 1;
     const uint64_t page_slots = ((uint64_t)1 << kRefPageShift) * S.ref_cap;
-    auto host_runs = [&](uint32_t q) { return net->host_runs + (uint64_t)q * page_slots; };
+    const uint64_t page_runs = page_slots * S.run_cap;  // a page's learned-pattern runs
+    auto host_runs = [&](uint32_t q) { return net->host_runs + (uint64_t)q * page_runs; };
     auto host_meta = [&](uint32_t q) { return net->host_meta + (uint64_t)q * page_slots; };
-    Runs* dr = nullptr;
+    Run* dr = nullptr;
     RefMeta* dm = nullptr;
     check(hipHostGetDevicePointer((void**)&dr, net->host_runs, 0), "hipHostGetDevicePointer");
     check(hipHostGetDevicePointer((void**)&dm, net->host_meta, 0), "hipHostGetDevicePointer");
@@ -3035,11 +3039,11 @@ void update_residency(snn_net* net) {
     for (uint32_t slot = 0; slot < nc; ++slot) {  // evictions
         const int32_t q = net->page_of_slot[slot];
         if (q < 0 || want[q]) continue;
-        check(hipMemcpyAsync(host_runs(q), net->cache_runs + slot * page_slots, page_slots * sizeof(Runs),
+        check(hipMemcpyAsync(host_runs(q), net->cache_runs + slot * page_runs, page_runs * sizeof(Run),
                              hipMemcpyDeviceToHost, net->stream), "page out");
         check(hipMemcpyAsync(host_meta(q), net->cache_meta + slot * page_slots, page_slots * sizeof(RefMeta),
                              hipMemcpyDeviceToHost, net->stream), "page out");
-        net->table[q] = RefPage{dr + (uint64_t)q * page_slots, dm + (uint64_t)q * page_slots};
+        net->table[q] = RefPage{dr + (uint64_t)q * page_runs, dm + (uint64_t)q * page_slots};
         net->slot_of_page[q] = -1;
         net->page_of_slot[slot] = -1;
         moved = true;
@@ -3050,11 +3054,11 @@ void update_residency(snn_net* net) {
         const uint32_t q = order[k];
         if (net->slot_of_page[q] >= 0) continue;
         while (net->page_of_slot[free_slot] >= 0) ++free_slot;
-        check(hipMemcpyAsync(net->cache_runs + free_slot * page_slots, host_runs(q), page_slots * sizeof(Runs),
+        check(hipMemcpyAsync(net->cache_runs + free_slot * page_runs, host_runs(q), page_runs * sizeof(Run),
                              hipMemcpyHostToDevice, net->stream), "page in");
         check(hipMemcpyAsync(net->cache_meta + free_slot * page_slots, host_meta(q), page_slots * sizeof(RefMeta),
                              hipMemcpyHostToDevice, net->stream), "page in");
-        net->table[q] = RefPage{net->cache_runs + free_slot * page_slots, net->cache_meta + free_slot * page_slots};
+        net->table[q] = RefPage{net->cache_runs + free_slot * page_runs, net->cache_meta + free_slot * page_slots};
         net->slot_of_page[q] = (int32_t)free_slot;
         net->page_of_slot[free_slot] = (int32_t)q;
         moved = true;

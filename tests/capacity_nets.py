@@ -19,6 +19,11 @@ def candidate_runs(distinct_ticks=10):
     return _net([p, p], syn, cmds, "capacity_candidate_runs")
 
 
+def candidate_runs_wide(distinct_ticks=34):
+    """candidate_runs for a network created with pattern_runs=32 (32 runs stored)."""
+    return candidate_runs(distinct_ticks)
+
+
 def learned_patterns(max_refs=5, observations=6):
     """Neuron 0 learns observations - 1 patterns in one window and keeps the last max_refs."""
     p = (100.0, 100, NEVER, max_refs)

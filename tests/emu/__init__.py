@@ -26,7 +26,7 @@ def lib():
         P = ctypes.c_void_p
         L.emu_create.restype = P
         L.emu_create.argtypes = [ctypes.c_uint64] + [P] * 4 + [ctypes.c_uint64] + [P] * 10 + \
-            [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_uint64]
+            [ctypes.c_uint64] * 5 + [ctypes.c_char_p, ctypes.c_uint64]
         L.emu_destroy.argtypes = [P]
         L.emu_inject.argtypes = [P, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float]
         L.emu_run_until.argtypes = [P, ctypes.c_uint64]
@@ -40,7 +40,7 @@ def lib():
         L.emu_synapses.argtypes = [P] * 5
         L.emu_sort_by_g.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         L.emu_temporal_check.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32,
-                                         ctypes.c_uint32]
+                                         ctypes.c_uint32, ctypes.c_uint32]
         L.emu_temporal_check.restype = ctypes.c_uint32
         L.emu_pack_check.argtypes = [ctypes.c_uint64] + [P] * 4 + [ctypes.c_uint64] + [P] * 10 + \
             [ctypes.c_uint32, ctypes.c_char_p, ctypes.c_uint64]
@@ -58,7 +58,7 @@ def _p(a):
 class Emu:
     """The device step's logic on the CPU, with NetworkPropagator-like calls."""
 
-    def __init__(self, net, ev_cap=1 << 20, fire_cap=1 << 16, seed=1, ref_cap=0):
+    def __init__(self, net, ev_cap=1 << 20, fire_cap=1 << 16, seed=1, ref_cap=0, pattern_runs=0):
         L = lib()
         self._keep = [np.ascontiguousarray(a) for a in (
             net.threshold, net.window, net.sim_thr, net.max_refs, net.src, net.tgt, net.weight,
@@ -67,7 +67,7 @@ class Emu:
         err = ctypes.create_string_buffer(512)
         self.n, self.e = net.n_neurons, net.n_synapses
         self.h = L.emu_create(self.n, *[_p(a) for a in k[:4]], self.e, *[_p(a) for a in k[4:]],
-                              ev_cap, fire_cap, seed, ref_cap, err, 512)
+                              ev_cap, fire_cap, seed, ref_cap, pattern_runs, err, 512)
         if not self.h:
             raise ValueError(err.value.decode())
         self.flags = 0
@@ -121,13 +121,13 @@ def sort_by_g(words, narrow_g, small=False):
     return a
 
 
-def temporal_check(ticks, times, window, sim, max_refs, cap):
+def temporal_check(ticks, times, window, sim, max_refs, cap, runs=8):
     """Index of the first tick where the per-tick temporal pattern update differs from the same
     observations applied one at a time, with a ring of `cap` learned-pattern slots (len(ticks) when
     they agree)."""
     ticks = np.ascontiguousarray(ticks, np.uint32)
     times = np.ascontiguousarray(times, np.uint32)
-    return lib().emu_temporal_check(_p(ticks), _p(times), len(ticks), window, sim, max_refs, cap)
+    return lib().emu_temporal_check(_p(ticks), _p(times), len(ticks), window, sim, max_refs, cap, runs)
 
 
 def pack_check(net, threads):

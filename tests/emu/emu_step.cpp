@@ -35,8 +35,8 @@ struct Emu {
     std::vector<uint32_t> pre_count, last_fired;
     std::vector<NeuronDyn> dyn;
     std::vector<TemporalHdr> th;
-    std::vector<Runs> cand;
-    std::vector<Runs> ref_runs;
+    std::vector<Run> cand;
+    std::vector<Run> ref_runs;
     std::vector<RefMeta> ref_meta;
     std::vector<float> decay, ltp, ltd;
     std::vector<uint32_t> firelog, firemult, tick_fcount;
@@ -271,8 +271,8 @@ extern "C" {
 void* emu_create(uint64_t n, const float* thr, const uint64_t* win, const float* sim, const uint64_t* mref, uint64_t e,
                  const uint32_t* src, const uint32_t* tgt, const float* w, const float* mw, const uint32_t* dl,
                  const uint8_t* comp, const uint8_t* rec, const uint8_t* pl, const uint64_t* cptr, const uint32_t* codes,
-                 uint64_t ev_cap, uint64_t fire_cap, uint64_t seed, uint64_t ref_cap, char* errbuf,
-                 uint64_t errcap) {
+                 uint64_t ev_cap, uint64_t fire_cap, uint64_t seed, uint64_t ref_cap, uint64_t run_cap,
+                 char* errbuf, uint64_t errcap) {
     try {
         NetDesc d{n, thr, win, sim, mref, e, src, tgt, w, mw, dl, comp, rec, pl, cptr, codes};
         Emu* m = new Emu();
@@ -318,12 +318,14 @@ void* emu_create(uint64_t n, const float* thr, const uint64_t* win, const float*
         S.thr_all = h.params[0].threshold;
         m->dyn.assign(h.n, NeuronDyn{});
         m->th.assign(h.n, TemporalHdr{});
-        m->cand.assign(h.n, Runs{});
+        S.run_cap = run_cap ? (uint32_t)run_cap : kRuns;
+        if (S.run_cap > kRunsMax) throw std::invalid_argument("pattern_runs above 64");
+        m->cand.assign((uint64_t)h.n * S.run_cap, Run{});
         // learned-pattern slots per neuron: the largest max_reference_patterns, or ref_cap
         uint64_t want = 1;
         for (const auto& q : h.params) want = std::max<uint64_t>(want, q.max_refs);
         S.ref_cap = (uint32_t)std::min<uint64_t>(want, ref_cap ? ref_cap : 4096);
-        m->ref_runs.assign((uint64_t)h.n * S.ref_cap, Runs{});
+        m->ref_runs.assign((uint64_t)h.n * S.ref_cap * S.run_cap, Run{});
         m->ref_meta.assign((uint64_t)h.n * S.ref_cap, RefMeta{});
         m->last_fired.assign(h.n, kNone);
         S.dyn = m->dyn.data();
@@ -518,27 +520,27 @@ void emu_sort_by_g(uint32_t* words, uint32_t k, uint32_t narrow_g, uint32_t widt
 // Returns the index of the first tick after which the two differ (header, error flags, stored
 // candidate runs, or any ring slot), or m when they agree throughout.
 uint32_t emu_temporal_check(const uint32_t* ticks, const uint32_t* times, uint32_t m, uint32_t window, float sim,
-                            uint32_t max_refs, uint32_t cap) {
+                            uint32_t max_refs, uint32_t cap, uint32_t runs) {
     NeuronParams p{1.0f, window, sim, max_refs};
     TemporalHdr ha{}, hb{};
-    Runs ca{}, cb{};
-    std::vector<Runs> rra(cap), rrb(cap);
+    std::vector<Run> ca(runs), cb(runs);
+    std::vector<Run> rra((uint64_t)cap * runs), rrb((uint64_t)cap * runs);
     std::vector<RefMeta> rma(cap), rmb(cap);
-    const RefsView ra{rra.data(), rma.data(), cap}, rb{rrb.data(), rmb.data(), cap};
+    const RefsView ra{rra.data(), rma.data(), cap, runs}, rb{rrb.data(), rmb.data(), cap, runs};
     for (uint32_t i = 0; i < m; ++i) {
-        const uint32_t ea = temporal_observe_tick(ha, &ca, ra, ticks[i], times[i], p);
+        const uint32_t ea = temporal_observe_tick(ha, ca.data(), ra, ticks[i], times[i], p);
         uint32_t eb = 0;
-        Runs* pc = &cb;
+        Run* pc = cb.data();
         bool dirty = false;
         for (uint32_t t = 0; t < times[i]; ++t) eb |= temporal_observe(hb, pc, rb, ticks[i], p, dirty);
         bool same = ea == eb && std::memcmp(&ha, &hb, sizeof ha) == 0;
-        for (uint32_t r = 0; same && r < ha.nruns; ++r) same = ca.val[r] == cb.val[r] && ca.cnt[r] == cb.cnt[r];
+        for (uint32_t r = 0; same && r < ha.nruns; ++r) same = ca[r].val == cb[r].val && ca[r].cnt == cb[r].cnt;
         // the learned patterns in ring order (slots outside the ring are never read)
         for (uint32_t q = 0; same && q < ha.nlearned; ++q) {
             const uint32_t k = ((ha.head_last & kHeadMask) + q) % cap;
             same = rma[k].len == rmb[k].len && rma[k].nruns == rmb[k].nruns;
             for (uint32_t r = 0; same && r < rma[k].nruns; ++r)
-                same = rra[k].val[r] == rrb[k].val[r] && rra[k].cnt[r] == rrb[k].cnt[r];
+                same = ra.at(k)[r].val == rb.at(k)[r].val && ra.at(k)[r].cnt == rb.at(k)[r].cnt;
         }
         if (!same) return i;
     }

@@ -52,10 +52,11 @@ def test_per_target_sort_orders_by_g_at_every_size(narrow_g, width, small):
             assert np.array_equal(out, rows[np.argsort(g, kind="stable")]), k
 
 
+@pytest.mark.parametrize("runs", [8, 32])
 @pytest.mark.parametrize("sim", [0.9, 1.0, 0.5, 0.0, -0.5])
 @pytest.mark.parametrize("max_refs,cap", [(1, 1), (1, 4), (2, 4), (4, 4), (5, 4), (3, 7), (9, 9), (40, 64),
                                           (500, 500), (500, 37)])
-def test_per_tick_temporal_update_equals_single_observations(sim, max_refs, cap):
+def test_per_tick_temporal_update_equals_single_observations(sim, max_refs, cap, runs):
     # The fold applies one neuron's k observations of a tick together (a window restart in closed
     # form, repeated matches in bulk); it must leave the header, candidate, learned ring and error
     # flags exactly as k calls of the one-observation update. Sparse and dense ticks, long gaps
@@ -69,7 +70,14 @@ def test_per_tick_temporal_update_equals_single_observations(sim, max_refs, cap)
         gaps = rng.choice([1, 1, 1, 2, 5, window + 1, window + 7], size=m)
         ticks = np.cumsum(gaps).astype(np.uint32)
         times = rng.choice([1, 1, 2, 3, 7, 12, 30], size=m).astype(np.uint32)
-        assert temporal_check(ticks, times, window, sim, max_refs, cap) == m, (trial, window)
+        assert temporal_check(ticks, times, window, sim, max_refs, cap, runs) == m, (trial, window)
+
+
+def test_pattern_runs_above_the_maximum_is_refused():
+    from tests import capacity_nets as C
+    with pytest.raises(ValueError, match="pattern_runs"):
+        Emu(C.candidate_runs(), pattern_runs=65)
+    Emu(C.candidate_runs(), pattern_runs=64)
 
 
 def test_network_too_large_for_one_device_is_refused_at_create():
@@ -107,6 +115,7 @@ def test_a_clamp_bound_per_plastic_synapse_matches_the_oracle():
 
 @pytest.mark.parametrize("case,limit,under,flag", [
     ("candidate_runs", {}, {"distinct_ticks": 8}, 8),      # 8 stored runs of a candidate
+    ("candidate_runs_wide", {"pattern_runs": 32}, {"distinct_ticks": 32}, 8),  # 32 with pattern_runs=32
     ("learned_patterns", {"ref_cap": 4}, {"max_refs": 4}, 8),  # the neuron's learned-pattern slots
     ("code_patterns", {}, {"fires": 5}, 16),                # 4 patterns of a synapse's code record
 ])

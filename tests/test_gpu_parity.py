@@ -155,6 +155,7 @@ def test_a_clamp_bound_per_plastic_synapse_matches_the_oracle():
 
 @pytest.mark.parametrize("case,limit,under,name", [
     ("candidate_runs", {}, {"distinct_ticks": 8}, "neuron-temporal-pattern"),
+    ("candidate_runs_wide", {"pattern_runs": 32}, {"distinct_ticks": 32}, "neuron-temporal-pattern"),
     ("learned_patterns", {"learned_pattern_slots": 4}, {"max_refs": 4}, "neuron-temporal-pattern"),
     ("code_patterns", {}, {"fires": 5}, "synapse-code-pattern"),
 ])
