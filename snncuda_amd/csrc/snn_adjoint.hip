@@ -153,8 +153,11 @@ struct TickMap {
 };
 
 // Reads the window's events from position p (at most up to end) in batches of 64, one event per
-// lane; lane j applies the batch's events of tick t0 + j in order. Returns the position after the
-// window's events.
+// lane; lane j applies the batch's events of tick t0 + j in order. kPrefetch batches (and their
+// weights) are requested at once, so a window costs one or two round trips to memory rather than
+// one per batch; batches past the window's end are read and not used. Returns the position after
+// the window's events.
+constexpr uint32_t kPrefetch = 4;
 __device__ uint64_t window_maps(const AdjArgs& a, uint64_t p, uint64_t end, uint32_t t0, uint32_t tw, uint32_t lane,
                                 WaveLds& L, TickMap& m) {
     for (int k = 0; k < 4; ++k) {
@@ -165,51 +168,63 @@ __device__ uint64_t window_maps(const AdjArgs& a, uint64_t p, uint64_t end, uint
     m.lr = 0;
     m.lastr = 0;
     const uint64_t pw = p;
-    while (true) {
-        const uint64_t q = p + lane;
-        uint32_t d = kWave;  // not in the window
-        AdjEvent e{};
-        if (q < end) {
-            e = a.ev[q];
-            d = (uint32_t)e.tick - t0;  // events before the window were consumed, so tick >= t0
+    bool more = true;
+    while (more) {
+        AdjEvent eb[kPrefetch];
+        float ub[kPrefetch];
+#pragma unroll
+        for (uint32_t b = 0; b < kPrefetch; ++b) {
+            const uint64_t q = p + b * kWave + lane;
+            eb[b] = AdjEvent{};
+            if (q < end) eb[b] = a.ev[q];
         }
-        const bool in = d < tw;
-        const uint32_t cnt = (uint32_t)__popcll(__ballot(in));  // a prefix: events are sorted by tick
-        L.d[lane] = in ? d : kWave;
-        L.lo[lane] = 0;
-        L.hi[lane] = 0;
-        if (in) {
-            L.u[lane] = ev_input(e, a.w);
-            L.md[lane] = e.meta;
-        }
-        wave_sync();
-        if (in) {
-            if (lane == 0 || L.d[lane - 1] != d) L.lo[d] = lane;
-            if (lane + 1 == kWave || L.d[lane + 1] != d) L.hi[d] = lane + 1;
-        }
-        wave_sync();
-        const uint32_t lo = L.lo[lane], hi = L.hi[lane];
-        for (uint32_t x = lo; x < hi; ++x) {
-            const uint32_t md = L.md[x];
-            const float u = L.u[x];
-            const uint32_t k = md & 3u, r = (md >> 2) & 3u;
-            m.eb[0] = k == 0 ? m.eb[0] + u : m.eb[0];
-            m.eb[1] = k == 1 ? m.eb[1] + u : m.eb[1];
-            m.eb[2] = k == 2 ? m.eb[2] + u : m.eb[2];
-            m.eb[3] = k == 3 ? m.eb[3] + u : m.eb[3];
-            m.wr |= 1u << k;
-            m.lr = (m.lr & ~(3u << (2 * k))) | (r << (2 * k));
-            if (md & 32u) {
-                for (int c = 0; c < 4; ++c) {
-                    m.ea[c] = 0.0f;
-                    m.eb[c] = 0.0f;
-                }
-                m.lastr = (uint32_t)(p + x - pw) + 1;
+#pragma unroll
+        for (uint32_t b = 0; b < kPrefetch; ++b) ub[b] = p + b * kWave + lane < end ? ev_input(eb[b], a.w) : 0.0f;
+#pragma unroll
+        for (uint32_t b = 0; b < kPrefetch; ++b) {
+            if (!more) break;  // uniform: cnt below is a wave-wide count
+            // every batch before this one was full, so p is the round's start + b * 64
+            const uint64_t q = p + lane;
+            uint32_t d = kWave;  // not in the window
+            if (q < end) d = (uint32_t)eb[b].tick - t0;  // events before the window were consumed: tick >= t0
+            const bool in = d < tw;
+            const uint32_t cnt = (uint32_t)__popcll(__ballot(in));  // a prefix: events are sorted by tick
+            L.d[lane] = in ? d : kWave;
+            L.lo[lane] = 0;
+            L.hi[lane] = 0;
+            if (in) {
+                L.u[lane] = ub[b];
+                L.md[lane] = eb[b].meta;
             }
+            wave_sync();
+            if (in) {
+                if (lane == 0 || L.d[lane - 1] != d) L.lo[d] = lane;
+                if (lane + 1 == kWave || L.d[lane + 1] != d) L.hi[d] = lane + 1;
+            }
+            wave_sync();
+            const uint32_t lo = L.lo[lane], hi = L.hi[lane];
+            for (uint32_t x = lo; x < hi; ++x) {
+                const uint32_t md = L.md[x];
+                const float u = L.u[x];
+                const uint32_t k = md & 3u, r = (md >> 2) & 3u;
+                m.eb[0] = k == 0 ? m.eb[0] + u : m.eb[0];
+                m.eb[1] = k == 1 ? m.eb[1] + u : m.eb[1];
+                m.eb[2] = k == 2 ? m.eb[2] + u : m.eb[2];
+                m.eb[3] = k == 3 ? m.eb[3] + u : m.eb[3];
+                m.wr |= 1u << k;
+                m.lr = (m.lr & ~(3u << (2 * k))) | (r << (2 * k));
+                if (md & 32u) {
+                    for (int c = 0; c < 4; ++c) {
+                        m.ea[c] = 0.0f;
+                        m.eb[c] = 0.0f;
+                    }
+                    m.lastr = (uint32_t)(p + x - pw) + 1;
+                }
+            }
+            wave_sync();
+            p += cnt;
+            if (cnt < kWave) more = false;
         }
-        wave_sync();
-        p += cnt;
-        if (cnt < kWave) break;
     }
     return p;
 }
