@@ -416,19 +416,22 @@ __global__ void __launch_bounds__(kBlock) k_adj_weight_grad(const uint64_t* sptr
     }
 }
 
-// sum of the per-neuron partials in FP64, one block, fixed order
-__global__ void __launch_bounds__(kBlock) k_adj_reduce(const double* part, uint32_t n, double* out) {
-    __shared__ double sm[kBlock][8];
+// sum of the per-neuron partials in FP64, one block, fixed order: thread t sums neurons t, t + 1024,
+// ... in order, then a pairwise tree over the threads (the same shape every call)
+constexpr int kReduceBlock = 1024;
+__global__ void __launch_bounds__(kReduceBlock) k_adj_reduce(const double* part, uint32_t n, double* out) {
+    __shared__ double sm[kReduceBlock][8];
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
         for (int q = 0; q < 8; ++q) acc[q] += part[(uint64_t)i * 8 + q];
     for (int q = 0; q < 8; ++q) sm[threadIdx.x][q] = acc[q];
     __syncthreads();
-    if (threadIdx.x < 8) {
-        double s = 0.0;
-        for (int t = 0; t < kBlock; ++t) s += sm[t][threadIdx.x];
-        out[threadIdx.x] = s;
+    for (uint32_t h = kReduceBlock / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int q = 0; q < 8; ++q) sm[threadIdx.x][q] += sm[threadIdx.x + h][q];
+        __syncthreads();
     }
+    if (threadIdx.x < 8) out[threadIdx.x] = sm[0][threadIdx.x];
 }
 
 struct DBuf {
@@ -751,7 +754,8 @@ int snn_adjoint_backward(snn_adjoint* adj, const float* d_grad_trace, float* d_g
                                (const float*)a.gev, adj->e, d_grad_weight);
             acheck(hipGetLastError(), "k_adj_weight_grad");
         }
-        hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(kBlock), 0, adj->stream, (const double*)a.part, adj->n, adj->sums);
+        hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(kReduceBlock), 0, adj->stream, (const double*)a.part, adj->n,
+                           adj->sums);
         acheck(hipMemcpyAsync(grad_params, adj->sums, 8 * sizeof(double), hipMemcpyDeviceToHost, adj->stream), "read");
         acheck(hipStreamSynchronize(adj->stream), "adjoint backward");
         return SNN_OK;
