@@ -25,6 +25,9 @@ The JSON line carries:
                  fold's own byte model.
   scatter_roofline, step_roofline   the partition scatter, and the whole tick (same byte model / tick time).
   stdp_on        the same measurement with STDP on (excitatory synapses plastic, threshold 2.75).
+  dense_regime   the same measurement (STDP off) at threshold 2.4, after 1,300 settle ticks: the
+                 activity grows to ~0.9% of the neurons firing per tick (~1.9e7 events per tick) and
+                 holds there, so the per-tick fixed costs spread over about twice the events.
   forms          the batched form (512 uneven networks stepped together vs looping the plain step
                  over a sample of them), the fixed-spike adjoint, and the spike-code network (100,000
                  neurons, 4 code offsets per synapse, STDP on; its whole 10,000-tick run), measured in
@@ -73,6 +76,10 @@ FOLD_STDP_RECORD_BYTES = 2 * 32.0
 # per-source record; STDP on every synapse
 SPIKECODE_BYTES_PER_EVENT = 10.0 + 8.0 + 2 * 96.0 + STDP_EVENT_BYTES
 SPIKECODE_THRESHOLD = 4.6
+# the denser regime of the 1M-neuron workload: from threshold 2.4 the activity grows for ~1,200 ticks
+# to about 0.9% of the neurons firing per tick (~1.9e7 events) and holds there (2.3 runs away)
+DENSE_THRESHOLD = 2.4
+DENSE_SETTLE_TICKS = 1300
 
 
 def parse():
@@ -81,10 +88,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=200, help="timed ticks")
     ap.add_argument("--warmup", type=int, default=50, help="untimed ticks after the settle phase")
     ap.add_argument("--neurons", type=int, default=1_000_000)
-    ap.add_argument("--form", choices=("all", "plain", "stdp", "batched", "adjoint", "spikecode"), default="all",
-                    help="all (default): the plain step as the headline line with STDP on, the batched form, the "
-                         "adjoint, the spike-code network and the CPU baselines measured in the same run; plain / "
-                         "stdp: one plain-step line; batched / adjoint / spikecode: one line of that form")
+    ap.add_argument("--form", choices=("all", "plain", "stdp", "dense", "batched", "adjoint", "spikecode"),
+                    default="all",
+                    help="all (default): the plain step as the headline line with STDP on, the denser regime, the "
+                         "batched form, the adjoint, the spike-code network and the CPU baselines measured in the same "
+                         "run; plain / stdp / dense: one plain-step line; batched / adjoint / spikecode: one line of "
+                         "that form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true",
                     help="one more pass of --steps ticks with HIP events around every kernel (kernel_ms_per_tick)")
@@ -112,16 +121,17 @@ def log(msg):
 
 # --------------------------------------------------------------------------- plain step
 
-def plain(args, stdp: bool) -> dict:
+def plain(args, stdp: bool, threshold: float = 0.0, settle: int = SETTLE_TICKS) -> dict:
     import numpy as np
     import snncuda_amd as sa
     from snncuda_amd import workloads as W
 
     n = args.neurons
-    total = SETTLE_TICKS + args.warmup + args.steps + (args.steps if args.breakdown else 0)
+    theta = threshold or threshold_of(stdp)
+    total = settle + args.warmup + args.steps + (args.steps if args.breakdown else 0)
     # STDP on: every synapse but those of inhibitory neurons is plastic (the rule caps weights at
     # 2.0, below the pinned inhibitory magnitudes U[0,5), so plastic inhibition would collapse)
-    spec, cmds = W.config2_spec(0, n=n, ticks=total, threshold=threshold_of(stdp), w_inh=5.0, kick=0.02,
+    spec, cmds = W.config2_spec(0, n=n, ticks=total, threshold=theta, w_inh=5.0, kick=0.02,
                                 plastic="exc" if stdp else False)
     injects = [c for c in cmds if c[0] == 0]
     inj_ticks = np.array([t for _, _, t, _ in injects], dtype=np.int64)
@@ -136,7 +146,7 @@ def plain(args, stdp: bool) -> dict:
             if len(v) > 3:
                 kw["fold_threads"] = v[3]
             dev.set_schedule(**kw)
-        first = SETTLE_TICKS + args.warmup
+        first = settle + args.warmup
         dev.run_until(first - 1)
         # timed region: one synchronous call covering exactly --steps ticks, with HIP events around
         # the scatter and the fold on every PROFILE_EVERY-th tick
@@ -209,7 +219,8 @@ def plain(args, stdp: bool) -> dict:
         "events_per_tick": syn_tick,
         "fires_per_tick": fires_tick,
         "fire_fraction_per_tick": fires_tick / n,
-        "threshold": threshold_of(stdp),
+        "threshold": theta,
+        "settle_ticks": settle,
         "synapses": int(E),
         "neurons": n + spec["n_sources"],
         "plastic_fraction": pfrac,
@@ -244,8 +255,8 @@ def plain_line(args, r: dict, stdp: bool) -> dict:
                         f"STDP {'on (excitatory synapses)' if stdp else 'off'}",
             "neurons": r["neurons"],
             "synapses": r["synapses"],
-            "settle_ticks": SETTLE_TICKS,
-            "timed_ticks": [SETTLE_TICKS + args.warmup, SETTLE_TICKS + args.warmup + args.steps - 1],
+            "settle_ticks": r["settle_ticks"],
+            "timed_ticks": [r["settle_ticks"] + args.warmup, r["settle_ticks"] + args.warmup + args.steps - 1],
             "events_per_tick": r["events_per_tick"],
             "fires_per_tick": r["fires_per_tick"],
             "fire_fraction_per_tick": r["fire_fraction_per_tick"],
@@ -602,7 +613,10 @@ def main():
         return
     stdp = args.form == "stdp"
     t_all = time.perf_counter()
-    r = plain(args, stdp)
+    if args.form == "dense":
+        r = plain(args, False, DENSE_THRESHOLD, DENSE_SETTLE_TICKS)
+    else:
+        r = plain(args, stdp)
     log(f"plain{' stdp' if stdp else ''}: {r['value'] / 1e9:.2f} G events/s, {r['ms_per_step']:.4f} ms/tick, "
         f"{r['events_per_tick']:.3e} events/tick, fold {r['roofline']['mean_launch_ms'] * 1e3:.1f} us")
     line = plain_line(args, r, stdp)
@@ -618,6 +632,18 @@ def main():
                                        "threshold 2.75, same settle/warmup/timed ticks")
         if s.get("kernel_ms_per_tick"):
             line["stdp_on"]["kernel_ms_per_tick"] = s["kernel_ms_per_tick"]
+        d = plain(args, False, DENSE_THRESHOLD, DENSE_SETTLE_TICKS)
+        log(f"dense: {d['value'] / 1e9:.2f} G events/s, {d['ms_per_step']:.4f} ms/tick, "
+            f"{d['events_per_tick']:.3e} events/tick, fold {d['roofline']['mean_launch_ms'] * 1e3:.1f} us")
+        line["dense_regime"] = {k: d[k] for k in ("value", "ms_per_step", "events_per_tick", "fires_per_tick",
+                                                  "fire_fraction_per_tick", "threshold", "settle_ticks",
+                                                  "step_roofline", "ticks_per_s")}
+        # the traffic files were measured in the headline regime: no traffic figure here
+        line["dense_regime"]["roofline"] = dict(d["roofline"], traffic=None)
+        line["dense_regime"]["unit"] = "synaptic events/s"
+        line["dense_regime"]["workload"] = (
+            f"the same network (STDP off) at threshold {DENSE_THRESHOLD}: about 0.9% of the neurons fire per "
+            f"tick once settled, {DENSE_SETTLE_TICKS} settle ticks")
         line["forms"] = {"batched": batched(args), "adjoint": adjoint_child(args), "spikecode": spikecode(args)}
     if not args.no_cpu_baseline:
         head, others = cpu_baselines(args)
