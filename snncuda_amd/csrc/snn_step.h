@@ -164,6 +164,7 @@ struct Step {
     // kernel of the tick also finishes the previous tick's counters (stream launches)
     uint32_t tick;
     uint32_t fused_advance;
+    uint32_t narrow_max;         // ticks of at most this many events sort by 32-bit keys (2^27)
     // Code patterns kept per source neuron (code_lazy): with one code offset per synapse and the
     // same window / similarity / max_refs on every target, a synapse's code pattern is a function of
     // its source's fire ticks shifted by the synapse's lag, so one record per source, advanced when a

@@ -701,7 +701,7 @@ __device__ __forceinline__ void fold_rounds(const Step& S, E* lds_stage, uint32_
     const uint32_t p = blockIdx.x;
     const E* src = (const E*)S.prec + start;
     const uint16_t* tls = S.ptl + start;
-    const bool narrow_g = S.seg_count[1] <= (1u << 27);
+    const bool narrow_g = S.seg_count[1] <= S.narrow_max;
     uint32_t err = 0;
     uint32_t run = 0;
     for (uint32_t t0 = 0; t0 < PS; t0 += kScanPer * BLOCK) {
@@ -775,7 +775,7 @@ __device__ __forceinline__ void fold_chunks(const Step& S, E* lds_stage, uint32_
     static_assert(kCap % BLOCK == 0, "the stage holds whole rounds of records");
     const uint32_t PS = 1u << S.pshift;
     const uint32_t tpt = PS / BLOCK;  // 1, 2 or 4 (partition shift 10 to 12)
-    const bool narrow_g = S.seg_count[1] <= (1u << 27);
+    const bool narrow_g = S.seg_count[1] <= S.narrow_max;
     const uint32_t p = blockIdx.x;
     const uint32_t nwg = S.nwg;
     const E* src = (const E*)S.prec + start;
@@ -954,7 +954,7 @@ __global__ void __launch_bounds__(BLOCK) k_pfold(Step S) {
     }
     __syncthreads();
     uint32_t err = 0;
-    const bool narrow_g = S.seg_count[1] <= (1u << 27);
+    const bool narrow_g = S.seg_count[1] <= S.narrow_max;
     if (c <= kCap) {
         // Sparse partition: every event fits the stage at once, so one round folds the touched
         // targets only, one per thread. One scan gives both the targets' places (low 16 bits) and
@@ -2326,6 +2326,11 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     }
     // code pattern records: per source when every synapse has one code offset and every target the
     // same temporal parameters (SNNCUDA_CODE_PER_SYNAPSE=1 keeps them per synapse), else per synapse
+    // the per-target sorts take one 32-bit key per entry when every g of the tick is below 2^27;
+    // SNNCUDA_WIDE_G=1 sends every tick through the sorts of full entries (the path of ticks with
+    // more events), so that path can be checked on small networks
+    const char* wide = std::getenv("SNNCUDA_WIDE_G");
+    S.narrow_max = (wide && std::atoi(wide)) ? 0u : (1u << 27);
     const char* eager = std::getenv("SNNCUDA_CODE_PER_SYNAPSE");
     S.code_lazy = (net->code_here && net->code2 && S.tp_uniform && !(eager && std::atoi(eager))) ? 1u : 0u;
     if (S.code_lazy) S.src_code = dev_alloc<CodeRec2>(own, n);

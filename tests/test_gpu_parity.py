@@ -55,6 +55,17 @@ def test_device_matches_oracle_other_seeds(seed):
             assert np.array_equal(pl["ltp"] + pl["ltd"], ds["upd_count"]), (net.name, "ltp + ltd")
 
 
+def test_wide_g_sorts_equal_the_oracle(monkeypatch):
+    """Ticks of more than 2^27 events sort each target's events by full entries instead of 32-bit
+    keys. SNNCUDA_WIDE_G=1 sends every tick through that path: small networks (sparse and rounds
+    paths) and the 1M-neuron law at 8,000 neurons (chunked path, STDP off and on) still equal the
+    oracle field by field."""
+    monkeypatch.setenv("SNNCUDA_WIDE_G", "1")
+    test_device_matches_oracle_other_seeds(3)
+    test_bench_law_matches_oracle(False, 2.5)
+    test_bench_law_matches_oracle(True, 2.75)
+
+
 def test_capacity_overflow_raises():
     net = G.CASES["config0_burst"]()
     with pytest.raises(sa.CapacityError):
