@@ -2243,10 +2243,12 @@ void finish_net(snn_net* net, const snn_options& o, uint64_t nve, const std::vec
     const uint32_t n = S.n;
     {
         // the groups' absolute starts, when they take at most 1/16 of the free device memory
+        // (SNNCUDA_NO_GROUP_STARTS=1 leaves them out, so the path of larger networks can be checked)
         const uint64_t words = (uint64_t)n * (S.lmax + 1);
         size_t free_b = 0, total_b = 0;
         check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-        if (words * 8 <= free_b / 16) {
+        const char* no_gs = std::getenv("SNNCUDA_NO_GROUP_STARTS");
+        if (words * 8 <= free_b / 16 && !(no_gs && std::atoi(no_gs))) {
             uint64_t* gs = dev_alloc<uint64_t>(own, words, false);
             hipLaunchKernelGGL(k_group_starts, dim3(2048), dim3(kBlock), 0, net->stream, S.row, S.grp, (uint64_t)n,
                                S.lmax + 1, gs);

@@ -55,12 +55,16 @@ def test_device_matches_oracle_other_seeds(seed):
             assert np.array_equal(pl["ltp"] + pl["ltd"], ds["upd_count"]), (net.name, "ltp + ltd")
 
 
-def test_wide_g_sorts_equal_the_oracle(monkeypatch):
-    """Ticks of more than 2^27 events sort each target's events by full entries instead of 32-bit
-    keys. SNNCUDA_WIDE_G=1 sends every tick through that path: small networks (sparse and rounds
-    paths) and the 1M-neuron law at 8,000 neurons (chunked path, STDP off and on) still equal the
-    oracle field by field."""
-    monkeypatch.setenv("SNNCUDA_WIDE_G", "1")
+@pytest.mark.parametrize("knob", ["SNNCUDA_WIDE_G", "SNNCUDA_NO_GROUP_STARTS"])
+def test_large_network_paths_equal_the_oracle(monkeypatch, knob):
+    """Paths that only larger networks take, forced on small ones: SNNCUDA_WIDE_G=1 sorts each
+    target's events by full entries instead of 32-bit keys (ticks of more than 2^27 events), and
+    SNNCUDA_NO_GROUP_STARTS=1 finds each segment from the row start and the group offsets (networks
+    whose table of group starts would take more than 1/16 of free memory). Small networks (sparse
+    and rounds fold paths) and the 1M-neuron law at 8,000 neurons (chunked path, STDP off and on)
+    still equal the oracle field by field."""
+    print(f"{knob}=1", flush=True)
+    monkeypatch.setenv(knob, "1")
     test_device_matches_oracle_other_seeds(3)
     test_bench_law_matches_oracle(False, 2.5)
     test_bench_law_matches_oracle(True, 2.75)
