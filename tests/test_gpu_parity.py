@@ -70,6 +70,28 @@ def test_large_network_paths_equal_the_oracle(monkeypatch, knob):
     test_bench_law_matches_oracle(True, 2.75)
 
 
+@pytest.mark.parametrize("delay_max", [130, 300])
+def test_long_delays_plan_in_global_memory_and_equal_the_oracle(delay_max):
+    """Delays and code offsets past 127 ticks make the tick's group table (2 * lmax + 1 groups) too
+    large for LDS: one wave plans it in global memory (k_plan) and the segment kernel reads it
+    there. Every feature of the step with delays up to delay_max (code offsets added), field by
+    field against the oracle."""
+    net = W.feature_net(40 + delay_max, n=80, fanout=8, ticks=2 * delay_max + 60, delay_max=delay_max)
+    print(f"{net.name}: delays up to {delay_max}", flush=True)
+    ref = oracle.Oracle(net)
+    ref.run_commands(net.commands)
+    with run_device(net) as dev:
+        assert np.array_equal(dev.tick_counts()[0], ref.tick_counts()[0])
+        assert np.array_equal(dev.raster(), ref.fires()[1])
+        dn, rn = dev.neurons(), ref.neurons()
+        for k in ("membrane", "current", "spike_count", "t_match", "t_learned", "last_fired"):
+            assert np.array_equal(dn[k], rn[k]), k
+        ds, rs = dev.synapses(), ref.synapses()
+        for k in ("weight", "pre_count", "upd_count", "last_pre", "last_post", "c_match", "c_learned"):
+            assert np.array_equal(ds[k], rs[k].astype(ds[k].dtype)), k
+    assert ref.delivered > 0
+
+
 def test_capacity_overflow_raises():
     net = G.CASES["config0_burst"]()
     with pytest.raises(sa.CapacityError):
