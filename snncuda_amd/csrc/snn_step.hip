@@ -2071,9 +2071,11 @@ void run_ticks(snn_net* net, uint32_t t0, uint64_t k, Timer& tm, TickGraph& g) {
     flush_advance(net, tm);
 }
 
-// Online autotuning on the run's own ticks. Coordinate search, one schedule field at a time (graph
-// length, then scatter workgroups, then partition size) around the best so far; every candidate
-// runs `trial` ticks in each of two interleaved rounds, timed with HIP events on the stream. The
+// Online autotuning on the run's own ticks. Coordinate search, one schedule field at a time (fold
+// workgroup size, then scatter workgroups, then partition size, then graph length) around the best
+// so far; every candidate runs `trial` ticks in each of two interleaved rounds, timed with HIP
+// events on the stream, except that a candidate more than 25% behind the best after the first
+// round is not run again (its score is the minimum over its rounds, so it could not win). The
 // score is the time per delivered event when the trial delivered at least 1e5 events per tick
 // (throughput-bound: activity drifts between trials), else the time per tick (launch-bound); the
 // lowest score wins. Returns the ticks consumed (0 when the run is too short, in which case tuning
@@ -2125,13 +2127,15 @@ uint64_t autotune(snn_net* net, uint32_t t0, uint64_t nt, Timer& tm, TickGraph& 
     std::string log;
     try {
         for (int dim = 0; dim < 4; ++dim) {
-            const std::vector<Schedule> cands = dim == 0   ? graph_cands(best)
+            const std::vector<Schedule> cands = dim == 0   ? fold_cands(best)
                                                 : dim == 1 ? nwg_cands(best)
                                                 : dim == 2 ? part_cands(best)
-                                                           : fold_cands(best);
+                                                           : graph_cands(best);
             std::vector<float> t(cands.size(), 1e30f);
-            for (int round = 0; round < 2; ++round)
+            for (int round = 0; round < 2; ++round) {
+                const float lead = *std::min_element(t.begin(), t.end());
                 for (size_t c = 0; c < cands.size(); ++c) {
+                    if (round > 0 && t[c] > 1.25f * lead) continue;
                     apply_schedule(net, cands[c]);
                     if (cands[c].graph && trial >= cands[c].graph) ensure_graph(net, t0, g);
                     uint64_t d0 = 0, d1 = 0;
@@ -2149,6 +2153,7 @@ uint64_t autotune(snn_net* net, uint32_t t0, uint64_t nt, Timer& tm, TickGraph& 
                     t[c] = std::min(t[c], score);
                     used += trial;
                 }
+            }
             size_t k = 0;
             for (size_t c = 1; c < cands.size(); ++c)
                 if (t[c] < t[k]) k = c;
@@ -2974,6 +2979,8 @@ int snn_net_inject(snn_net* net, const uint32_t* neuron, const uint64_t* tick, c
     if (!net || (n && (!neuron || !tick || !weight))) return fail(SNN_EINVAL, "null argument");
     for (uint64_t i = 0; i < n; ++i)
         if (neuron[i] >= net->n) return fail(SNN_EINVAL, "inject neuron out of range");
+    if (net->pending.capacity() < net->pending.size() + n)  // geometric growth for many small calls
+        net->pending.reserve(std::max<size_t>(net->pending.size() + n, 2 * net->pending.capacity()));
     for (uint64_t i = 0; i < n; ++i) {
         // SpikeScheduler::schedule clamps a late event to the next poll tick
         const uint64_t t = std::max<uint64_t>(tick[i], net->next_tick);
@@ -3087,16 +3094,17 @@ int run_until_impl(snn_net* net, uint64_t inclusive_tick, bool static_staging) {
         check(hipSetDevice(net->device), "hipSetDevice");
         const uint64_t t0 = net->next_tick;
         const uint64_t nt = inclusive_tick - t0 + 1;
-        // injects delivered inside this segment, ordered by (tick, stage, call order)
-        std::vector<PendingInject> now, later;
-        for (const auto& p : net->pending) (p.tick <= inclusive_tick ? now : later).push_back(p);
-        std::sort(now.begin(), now.end(), [](const PendingInject& a, const PendingInject& b) {
-            if (a.tick != b.tick) return a.tick < b.tick;
-            if (a.stage != b.stage) return a.stage < b.stage;
-            return a.seq < b.seq;
-        });
+        // injects delivered inside this segment, ordered by (tick, stage, call order). `pending` is
+        // in call order and a call's stage (the clock when it was made) never decreases along it,
+        // so a stable counting sort by tick gives that order: iptr holds each tick's first place.
+        // Every pending tick is >= t0 (snn_net_inject clamps to the clock; earlier runs took the rest).
+        const std::vector<PendingInject>& pend = net->pending;
+        std::vector<uint64_t> iptr(nt + 1, 0);
+        for (const auto& p : pend)
+            if (p.tick <= inclusive_tick) iptr[p.tick - t0 + 1]++;
+        for (uint64_t t = 0; t < nt; ++t) iptr[t + 1] += iptr[t];
         // staging layout of snn_net_run_until_static, checked before anything changes
-        const size_t ni = now.size();
+        const size_t ni = iptr[nt];
         const size_t o_w = align256(ni * 4), o_stage = o_w + align256(ni * 4), o_ptr = o_stage + align256(ni * 4);
         const size_t o_del = o_ptr + align256((nt + 1) * 8), o_fir = o_del + align256(nt * 4);
         const size_t need = o_fir + align256(nt * 4);
@@ -3105,18 +3113,27 @@ int run_until_impl(snn_net* net, uint64_t inclusive_tick, bool static_staging) {
                                         std::to_string(ni) + " injects) needs " + std::to_string(need) +
                                         " bytes of staging; the static scratch holds " +
                                         std::to_string(kStaticScratchBytes) + " (use shorter segments)");
-        net->pending.swap(later);
-        if (net->record) net->inj_log.insert(net->inj_log.end(), now.begin(), now.end());
-        std::vector<uint32_t> itgt(now.size()), istage(now.size());
-        std::vector<float> iw(now.size());
-        std::vector<uint64_t> iptr(nt + 1, 0);
-        for (size_t i = 0; i < now.size(); ++i) {
-            itgt[i] = now[i].neuron;
-            iw[i] = now[i].weight;
-            istage[i] = (uint32_t)now[i].stage;
-            iptr[now[i].tick - t0 + 1]++;
+        std::vector<uint32_t> itgt(ni), istage(ni);
+        std::vector<float> iw(ni);
+        std::vector<uint64_t> cursor(iptr.begin(), iptr.end() - 1);
+        std::vector<size_t> order(net->record ? ni : 0);  // place -> index in pend, for the inject log
+        std::vector<PendingInject> later;
+        later.reserve(pend.size() - ni);
+        for (size_t i = 0; i < pend.size(); ++i) {
+            const PendingInject& p = pend[i];
+            if (p.tick > inclusive_tick) {
+                later.push_back(p);
+                continue;
+            }
+            const uint64_t k = cursor[p.tick - t0]++;
+            itgt[k] = p.neuron;
+            iw[k] = p.weight;
+            istage[k] = (uint32_t)p.stage;
+            if (net->record) order[k] = i;
         }
-        for (uint64_t t = 0; t < nt; ++t) iptr[t + 1] += iptr[t];
+        if (net->record)
+            for (size_t k = 0; k < ni; ++k) net->inj_log.push_back(pend[order[k]]);
+        net->pending.swap(later);
         Step& S = net->S;
         if (static_staging) {
             uint8_t* base = scratch_on(net->device);

@@ -114,6 +114,39 @@ def test_split_runs_equal_one_run():
     b.close()
 
 
+@pytest.mark.parametrize("seed", [2, 9])
+def test_injects_out_of_tick_order_from_several_calls_equal_the_oracle(seed):
+    """Injects issued in shuffled tick order, split over calls made at different clock positions, so
+    one tick receives injects of several calls (ordered by the call's clock, then by call order) and
+    late ones are clamped to the clock; the run is cut into segments of uneven length. Threshold
+    1.2 keeps the clamped bursts from igniting the network (at 1.1 they do, and activity grows
+    without bound)."""
+    print(f"seed {seed}", flush=True)
+    rng = np.random.default_rng(seed)
+    base = W.config0(seed, threshold=1.2, ticks=260)
+    inj = [c for c in base.commands if c[0] == 0]
+    inj = [inj[i] for i in rng.permutation(len(inj))]
+    extra = [(0, int(rng.integers(0, base.n_neurons)), int(rng.integers(0, 260)),
+              float(np.float32(rng.uniform(0.05, 1.0)))) for _ in range(200)]
+    groups = np.array_split(np.arange(len(inj) + len(extra)), 4)
+    allc = inj + extra
+    cmds = []
+    for g, end in zip(groups, (37, 38, 150, 259)):
+        cmds += [allc[i] for i in g]
+        cmds.append((1, 0, end, 0.0))
+    ref = oracle.Oracle(base)
+    ref.run_commands(cmds)
+    with sa.Network(base) as dev:
+        dev.run_commands(cmds)
+        assert np.array_equal(dev.tick_counts()[0], ref.tick_counts()[0])
+        assert np.array_equal(dev.raster(), ref.fires()[1])
+        dn, rn = dev.neurons(), ref.neurons()
+        for k in ("membrane", "current", "spike_count", "last_fired"):
+            assert np.array_equal(dn[k], rn[k]), k
+        assert np.array_equal(dev.synapses()["weight"], ref.synapses()["weight"].astype(np.float32))
+    assert ref.fired > 250
+
+
 @pytest.mark.parametrize("plastic", [False, True, "exc"])
 def test_device_generator_equals_host_build(plastic):
     # snn_net_create_ei must build the same network as snn_net_create on workloads.config2's arrays
