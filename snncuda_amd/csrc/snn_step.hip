@@ -1054,31 +1054,41 @@ __global__ void __launch_bounds__(kBlock) k_fire_write(Step S) {
     const bool fits = head + F - oldest <= S.fire_cap;
     // the raster is a recording aid: when it is full, recording stops and the run goes on
     const bool rfits = S.raster && !(rflags & 1u) && rfill + F <= S.raster_cap;
+    // Each round's fire positions are listed in rank order in LDS first, then written by all
+    // threads: a word holding many fires (the inject events' words often do) would otherwise keep
+    // one thread walking its bits with a memory round trip per fire.
+    __shared__ uint32_t list[kBlock * 32];
     uint32_t base = before;
     for (uint32_t off = lo; off < hi; off += blockDim.x) {
         const uint32_t wi = off + threadIdx.x;
         uint32_t bits = next;
         next = wi + blockDim.x < hi ? S.fire_bits[wi + blockDim.x] : 0u;
         uint32_t tot;
-        uint32_t pos = base + block_excl_scan(__popc(bits), &tot);
-        base += tot;
-        if (!bits) continue;
-        S.fire_bits[wi] = 0;
+        uint32_t r = block_excl_scan(__popc(bits), &tot);
+        if (!tot) continue;  // block-uniform
+        if (bits) S.fire_bits[wi] = 0;
         while (bits) {
             const uint32_t b = __ffs(bits) - 1;
             bits &= bits - 1;
-            const uint32_t g = wi * 32 + b;
+            list[r++] = wi * 32 + b;
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < tot; j += blockDim.x) {
+            const uint32_t g = list[j];
+            const uint32_t x = S.fire_neuron[g];
+            const uint64_t pos = base + j;
             if (fits) {
                 const uint64_t idx = (head + pos) % S.fire_cap;
-                S.firelog[idx] = S.fire_neuron[g];
+                S.firelog[idx] = x;
                 S.firemult[idx] = S.fire_mult[g];
             }
             if (rfits) {
-                S.raster[rfill + pos] = S.fire_neuron[g];
+                S.raster[rfill + pos] = x;
                 if (S.raster_evt) S.raster_evt[rfill + pos] = S.fire_evt[g];
             }
-            ++pos;
         }
+        base += tot;
+        __syncthreads();  // the list is refilled by the next round
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         S.ctl[2] = F;
