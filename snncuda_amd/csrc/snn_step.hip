@@ -653,27 +653,39 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
         }
         if (ch == 0) return;
         __syncthreads();
-        // write out every full buffer: one wave per partition, one lane per record
-        for (uint32_t p = wave; p < np; p += nwaves) {
-            const uint32_t c = cnt[p];
-            if (c < ch) continue;
-            if (lane < ch) {
-                out[base[p] + lane] = buf[p * ch + lane];
-                outt[base[p] + lane] = buft[p * ch + lane];
-            }
-            if (lane == 0) {
-                base[p] += c;
-                cnt[p] = 0;
+        // write out every full buffer: wave w owns partitions w, w + nwaves, ...; its lanes check 64
+        // of them at once and the wave writes each full one, one lane per record
+        for (uint32_t k0 = 0; k0 * nwaves * kWave < np; ++k0) {
+            const uint32_t pl = wave + nwaves * (lane + kWave * k0);
+            uint64_t m = __ballot(pl < np && cnt[pl] >= ch);
+            while (m) {
+                const uint32_t p = wave + nwaves * ((uint32_t)__builtin_ctzll(m) + kWave * k0);
+                m &= m - 1;
+                const uint32_t c = cnt[p];
+                if (lane < ch) {
+                    out[base[p] + lane] = buf[p * ch + lane];
+                    outt[base[p] + lane] = buft[p * ch + lane];
+                }
+                if (lane == 0) {
+                    base[p] += c;
+                    cnt[p] = 0;
+                }
             }
         }
         __syncthreads();
     });
-    // the partial buffers
-    for (uint32_t p = wave; p < np; p += nwaves) {
-        const uint32_t c = min(cnt[p], ch);
-        if (lane < c) {
-            out[base[p] + lane] = buf[p * ch + lane];
-            outt[base[p] + lane] = buft[p * ch + lane];
+    // the partial buffers (the same ownership; only the partitions holding records are visited)
+    for (uint32_t k0 = 0; k0 * nwaves * kWave < np; ++k0) {
+        const uint32_t pl = wave + nwaves * (lane + kWave * k0);
+        uint64_t m = __ballot(pl < np && cnt[pl] > 0);
+        while (m) {
+            const uint32_t p = wave + nwaves * ((uint32_t)__builtin_ctzll(m) + kWave * k0);
+            m &= m - 1;
+            const uint32_t c = min(cnt[p], ch);
+            if (lane < c) {
+                out[base[p] + lane] = buf[p * ch + lane];
+                outt[base[p] + lane] = buft[p * ch + lane];
+            }
         }
     }
     if (err) atomicOr(&S.ctl[1], err);
