@@ -595,11 +595,13 @@ __global__ void __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(8)
         uint32_t run = 0;
         for (uint32_t p0 = 0; p0 < np; p0 += blockDim.x) {
             const uint32_t p = p0 + threadIdx.x;
+            // the workgroup's offset within the partition is requested with the total, before the scan
             const uint32_t v = p < np ? S.ptotal[p] : 0u;
+            const uint32_t own = p < np ? S.hist[(uint64_t)p * S.nwg + blockIdx.x] : 0u;
             uint32_t tot;
             const uint32_t ex = run + block_excl_scan_n<kPBlock>(v, &tot);
             if (p < np) {
-                base[p] = ex + S.hist[(uint64_t)p * S.nwg + blockIdx.x];
+                base[p] = ex + own;
                 cnt[p] = 0;
                 if (blockIdx.x == 0) S.pstart[p] = ex;  // for the fold
             }
