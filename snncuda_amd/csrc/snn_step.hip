@@ -380,6 +380,39 @@ __device__ __forceinline__ uint32_t seg_lookup(const SegCache& c, uint32_t ga, u
     return k;
 }
 
+// The last q in [0, n) with a[q] <= g, for a non-decreasing a with a[0] <= g. Up to kSerialFind
+// entries one thread's binary search (at most 12 dependent loads) is cheaper; above, the whole block
+// finds it in two parallel passes: a sample of up to blockDim.x evenly spaced entries, then the
+// entries between the two neighbouring samples that enclose the answer (those <= g are a prefix of
+// each). Every thread of the block calls it, once per kernel, and gets the result.
+constexpr uint32_t kSerialFind = 4096;
+__device__ __forceinline__ uint32_t block_last_le(const uint32_t* a, uint32_t n, uint32_t g) {
+    if (n <= kSerialFind) {  // block-uniform
+        __shared__ uint32_t found;
+        if (threadIdx.x == 0) {
+            uint32_t lo = 1, hi = n;  // a[0] <= g
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a[mid] <= g) lo = mid + 1; else hi = mid;
+            }
+            found = lo - 1;
+        }
+        __syncthreads();
+        return found;
+    }
+    const uint32_t B = blockDim.x, t = threadIdx.x;
+    const uint32_t m = min(B, n);
+    auto at = [&](uint32_t k) { return (uint32_t)((uint64_t)k * n / m); };  // sample k's index; at(m) = n
+    const uint32_t c1 = (uint32_t)__syncthreads_count(t < m && a[at(t)] <= g);  // >= 1: a[0] <= g
+    const uint32_t lo = at(c1 - 1), hi = c1 < m ? at(c1) : n;
+    uint32_t c2 = 0;
+    for (uint32_t q0 = lo + 1; q0 < hi; q0 += B) {
+        const uint32_t q = q0 + t;
+        c2 += (uint32_t)__syncthreads_count(q < hi && a[q] <= g);
+    }
+    return lo + c2;
+}
+
 // Per-workgroup histogram of the tick's events over target partitions (LDS atomics only).
 //
 // The tick's segment offsets have no scan kernel of their own: k_segments left each segment's first
@@ -428,27 +461,24 @@ __global__ void __launch_bounds__(kPBlock) k_pcount(Step S) {
     constexpr uint32_t kCountUnroll = 4;
     const uint32_t g0 = (uint32_t)(D * blockIdx.x / gridDim.x);
     const uint32_t g1 = (uint32_t)(D * (blockIdx.x + 1) / gridDim.x);
-    __shared__ uint32_t qstart;
+    // the last segment whose first event is <= g0: in the last block (of those holding segments)
+    // whose first event is <= g0, the last segment whose offset is <= g0
+    __shared__ uint32_t qblk;
     if (threadIdx.x == 0 && g0 < g1) {
-        // the last segment whose first event is <= g0: in the last block (of those holding
-        // segments) whose first event is <= g0, the last segment whose offset is <= g0
         const uint32_t nblk = (nseg + chunk - 1) / chunk;
         uint32_t a = 1, b = nblk;  // pre[0] = 0 <= g0
         while (a < b) {
             const uint32_t mid = (a + b) >> 1;
             if (pre[mid] <= g0) a = mid + 1; else b = mid;
         }
-        const uint32_t blk = a - 1;
-        const uint32_t r = g0 - pre[blk];
-        uint32_t x = blk * chunk + 1, y = min(nseg, blk * chunk + chunk);  // seg_loc of the first is 0
-        while (x < y) {
-            const uint32_t mid = (x + y) >> 1;
-            if (S.seg_loc[mid] <= r) x = mid + 1; else y = mid;
-        }
-        qstart = x - 1;
+        qblk = a - 1;
     }
     __syncthreads();
-    uint32_t qa = g0 < g1 ? qstart : nseg;
+    uint32_t qa = nseg;
+    if (g0 < g1) {  // block-uniform
+        const uint32_t first = qblk * chunk;  // its seg_loc is 0
+        qa = first + block_last_le(S.seg_loc + first, min(nseg, first + chunk) - first, g0 - pre[qblk]);
+    }
     while (g0 < g1) {
         const uint32_t qb = min(nseg, qa + kSegChunk);
         const uint32_t nq = qb - qa;
@@ -518,11 +548,8 @@ __device__ void for_block_rounds(const Step& S, SegCache& c, F&& f) {
     const uint64_t D = S.seg_count[1];
     const uint32_t g0 = (uint32_t)(D * blockIdx.x / gridDim.x);
     const uint32_t g1 = (uint32_t)(D * (blockIdx.x + 1) / gridDim.x);
-    __shared__ uint32_t qstart;
-    if (threadIdx.x == 0 && g0 < g1) qstart = seg_find(S.seg_off, nseg, g0);
-    __syncthreads();
-    if (g0 >= g1) return;
-    uint32_t qa = qstart;
+    if (g0 >= g1) return;  // block-uniform
+    uint32_t qa = block_last_le(S.seg_off, nseg, g0);  // seg_off[0] = 0 <= g0
     while (true) {
         const uint32_t qb = min(nseg, qa + kSegChunk);
         const uint32_t nq = qb - qa;
