@@ -220,6 +220,13 @@ SNN_HD RefsView refs_of(const Step& S, uint32_t x) {
 }
 
 SNN_HD uint32_t lower_bound_u32(const uint32_t* a, uint64_t lo, uint64_t hi, uint32_t v) {
+    // both ends first (two independent loads): a tick's injects usually share one stage, so the
+    // bound is usually one of them and the chain of dependent loads of the search is skipped
+    if (lo < hi) {
+        const uint32_t first = a[lo], last = a[hi - 1];
+        if (last < v) return (uint32_t)hi;
+        if (first >= v) return (uint32_t)lo;
+    }
     while (lo < hi) {
         const uint64_t mid = (lo + hi) >> 1;
         if (a[mid] < v) lo = mid + 1; else hi = mid;
